@@ -1,0 +1,49 @@
+/*
+ * Additional entry points of libpdhcg_amd.so (MI355X).  None of these exist
+ * in the reference interface; they are new symbols that leave the pdhcg.h ABI
+ * untouched.  They expose what the reference keeps internal (the solver state
+ * of reference src/solver.cu:35-210, stepped in chunks of iterations) so that
+ * benchmarks and tests can time the iteration without setup, and they select
+ * the arithmetic precision out of band (the parameter struct is ABI).
+ */
+#pragma once
+
+#include "pdhcg_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of the device iteration.  FP64: matrix values, vectors and
+ * SpMV accumulation in double (the reference's arithmetic).  MIXED: matrix
+ * values, vectors and SpMV accumulation in float; every reduction, the
+ * restart test, the primal weight and the KKT evaluation in double. */
+#define PDHCG_PRECISION_FP64 0
+#define PDHCG_PRECISION_MIXED 1
+
+/* Precision used by solve_qp_problem.  Defaults to FP64, or to MIXED when
+ * the environment variable PDHCG_PRECISION is "mixed" when first queried. */
+void pdhcg_set_precision(int precision);
+int pdhcg_get_precision(void);
+
+/* Sets or clears the cancellation flag checked at every KKT evaluation
+ * (termination reason USER_INTERRUPT), as the reference's SIGINT handler does
+ * through its global flag (reference src/pdhcg.c:31). */
+void pdhcg_request_cancel(int flag);
+
+/* Solver handle: scaling, device upload and step-size estimation happen in
+ * create; run performs up to `iterations` passes of the main loop body
+ * (evaluations, restarts and termination included) and returns the
+ * termination reason, 0 while still running, or -1 on a device error.
+ * The problem must outlive the handle. */
+typedef struct pdhcg_solver pdhcg_solver_t;
+pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params, int precision);
+int pdhcg_solver_run(pdhcg_solver_t *solver, int iterations);
+int pdhcg_solver_counts(pdhcg_solver_t *solver, int *total_iterations, int *inner_iterations);
+int pdhcg_solver_synchronize(pdhcg_solver_t *solver);
+pdhcg_result_t *pdhcg_solver_result(pdhcg_solver_t *solver);
+void pdhcg_solver_free(pdhcg_solver_t *solver);
+
+#ifdef __cplusplus
+}
+#endif

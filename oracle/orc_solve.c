@@ -1,0 +1,857 @@
+/*
+ * ORACLE (test infrastructure, see oracle.h): the restarted Halpern PDHG
+ * iteration with the projected Barzilai-Borwein inner solve, adaptive restart,
+ * PID primal weight, KKT residuals and infeasibility rays, in FP64 on the CPU.
+ * Restated step for step from reference src/solver.cu:35-210,
+ * src/pdhg_core_op.cu, src/kernels/pdhcg_kernels.cu and src/utils.cu:219-322.
+ * Reductions are plain sequential sums.
+ */
+#include "orc_internal.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+typedef enum { P_LP, P_QP } ptype_t;
+
+struct orc_solver {
+    pdhg_parameters_t params;
+    orc_scaled *w;
+    int n, m, k;
+    orc_csr At, Rt;
+    const qp_problem_t *orig;      /* caller problem (result dims) */
+    qp_problem_t *dummy;           /* owned when a dummy row was added */
+    double obj_norm, bound_norm;
+    double q_norm, nonconvexity;
+    ptype_t ptype;
+    /* iterates (scaled space) */
+    double *x0, *x, *xp, *xr, *y0, *y, *yp, *yr;
+    double *aty, *ax, *dslack, *pslack, *pres, *dres, *dx, *dy;
+    double *qx, *rx, *rxbuf;
+    double *cl_fin, *cu_fin, *lb_fin, *ub_fin;
+    /* BB inner solver (reference internal/internal_types.h:71-94) */
+    int bb;
+    double *grad, *dirn, *pbuf, *diag_h, *mdiag, *minv;
+    int precond;
+    double cached_inv_tau, tol_scale, inner_tol, inner_min_tol;
+    int inner_limit, inner_total;
+    /* scalars */
+    double step_size, primal_weight, pw_err_sum, pw_last_err, best_pw, best_gap;
+    double fpe, init_fpe, last_trial_fpe;
+    int total_count, inner_count, is_major, do_restart;
+    double abs_pr, rel_pr, abs_du, rel_du, pobj, dobj, gap, rel_gap;
+    double max_pray, max_dray, pray_lin_obj, dray_obj;
+    termination_reason_t reason;
+    double cum_time;
+    struct timespec t_start;
+};
+
+static void *xm(size_t n) { void *p = malloc(n ? n : 1); if (!p) abort(); return p; }
+static double *zeros(int n) { double *p = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); if (!p) abort(); return p; }
+
+/* y = M x over CSR */
+static void spmv(const orc_csr *M, const double *x, double *y) {
+    for (int r = 0; r < M->rows; ++r) {
+        double s = 0.0;
+        for (int e = M->rp[r]; e < M->rp[r + 1]; ++e) s += M->v[e] * x[M->ci[e]];
+        y[r] = s;
+    }
+}
+static double dot(int n, const double *a, const double *b) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+static double nrm2(int n, const double *a) { return sqrt(dot(n, a, a)); }
+/* reference src/utils.cu:548-567 (first index of max |x|; non-finite -> inf) */
+static double inf_norm(int n, const double *a) {
+    if (n <= 0) return 0.0;
+    double mx = -1.0;
+    int idx = 0;
+    for (int i = 0; i < n; ++i) {
+        double v = fabs(a[i]);
+        if (isnan(a[i])) return INFINITY;
+        if (v > mx) { mx = v; idx = i; }
+    }
+    if (!isfinite(a[idx])) return INFINITY;
+    return fabs(a[idx]);
+}
+static double clampd(double v, double lo, double hi) { return fmax(lo, fmin(v, hi)); }
+
+/* reference src/utils.cu:535-546 */
+static int print_freq(int iter) {
+    int step = 10;
+    long long thr = 1000;
+    while (iter >= thr) { step *= 10; thr *= 10; }
+    return step;
+}
+
+static void csr_transpose(const orc_csr *A, orc_csr *T) {
+    T->rows = A->cols; T->cols = A->rows; T->nnz = A->nnz;
+    T->rp = (int *)xm((size_t)(T->rows + 1) * sizeof(int));
+    T->ci = (int *)xm((size_t)A->nnz * sizeof(int));
+    T->v = (double *)xm((size_t)A->nnz * sizeof(double));
+    orc_csr_transpose(A->rows, A->cols, A->rp, A->ci, A->v, T->rp, T->ci, T->v);
+}
+
+/* ---- Q products: reference src/pdhg_core_op.cu:272-413 ---- */
+static void apply_middle(orc_solver_t *s) {
+    orc_scaled *w = s->w;
+    int k = s->k;
+    if (w->dkind == 0 || k <= 0) return;
+    if (w->dkind == 1) {
+        for (int r = 0; r < k; ++r) s->rx[r] *= w->ddiag[r];
+        return;
+    }
+    /* Dsymv with the lower triangle of the column-major view of the row-major
+       array, i.e. the upper triangle of the row-major D. */
+    for (int i = 0; i < k; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < k; ++j) {
+            int a = i < j ? i : j, b = i < j ? j : i;
+            acc += w->ddense[(size_t)a * k + b] * s->rx[j];
+        }
+        s->rxbuf[i] = acc;
+    }
+    memcpy(s->rx, s->rxbuf, (size_t)k * sizeof(double));
+}
+
+static void obj_product(orc_solver_t *s, const double *x) {
+    orc_scaled *w = s->w;
+    int n = s->n;
+    switch (w->qtype) {
+    case PDHCG_NON_Q:
+        return;
+    case PDHCG_DIAG_Q:
+        for (int i = 0; i < n; ++i) s->qx[i] = w->qdiag[i] * x[i];
+        break;
+    case PDHCG_SPARSE_Q:
+        spmv(&w->Q, x, s->qx);
+        break;
+    case PDHCG_LOW_RANK_Q:
+        spmv(&w->R, x, s->rx);
+        apply_middle(s);
+        spmv(&s->Rt, s->rx, s->qx);
+        break;
+    case PDHCG_LOW_RANK_PLUS_SPARSE_Q:
+        spmv(&w->Q, x, s->qx);
+        spmv(&w->R, x, s->rx);
+        apply_middle(s);
+        for (int r = 0; r < n; ++r) {
+            double acc = 0.0;
+            for (int e = s->Rt.rp[r]; e < s->Rt.rp[r + 1]; ++e) acc += s->Rt.v[e] * s->rx[s->Rt.ci[e]];
+            s->qx[r] += acc;
+        }
+        break;
+    }
+}
+
+/* ---- spectral estimates: reference src/pdhg_core_op.cu:1498-1788 ---- */
+static double est_max_eig(const orc_csr *Q, int maxit, double tol) {
+    int n = Q->rows;
+    double *v = zeros(n), *av = zeros(n);
+    unsigned int seed = 1234;
+    for (int i = 0; i < n; ++i) v[i] = (double)rand_r(&seed) / RAND_MAX;
+    double lam = 0.0;
+    for (int it = 0; it < maxit; ++it) {
+        double nv = nrm2(n, v), inv = 1.0 / nv;
+        for (int i = 0; i < n; ++i) v[i] *= inv;
+        spmv(Q, v, av);
+        double old = lam;
+        lam = dot(n, v, av);
+        if (it > 0 && fabs(lam - old) < tol) break;
+        memcpy(v, av, (size_t)n * sizeof(double));
+    }
+    free(v); free(av);
+    return lam;
+}
+
+static double est_min_eig(const orc_csr *Q, double lmax, int maxit, double tol) {
+    int n = Q->rows;
+    double *v = zeros(n), *av = zeros(n), *sh = zeros(n);
+    unsigned int seed = 1234;
+    for (int i = 0; i < n; ++i) v[i] = (double)rand_r(&seed) / RAND_MAX;
+    double mu = 0.0;
+    for (int it = 0; it < maxit; ++it) {
+        double nv = nrm2(n, v), inv = 1.0 / nv;
+        for (int i = 0; i < n; ++i) v[i] *= inv;
+        spmv(Q, v, av);
+        double old = mu;
+        for (int i = 0; i < n; ++i) sh[i] = -av[i] + lmax * v[i];
+        mu = dot(n, v, sh);
+        if (it > 0 && fabs(mu - old) < tol) break;
+        memcpy(v, sh, (size_t)n * sizeof(double));
+    }
+    free(v); free(av); free(sh);
+    return lmax - mu;
+}
+
+static double est_max_sv(const orc_csr *A, const orc_csr *At, int maxit, double tol) {
+    int m = A->rows, n = A->cols;
+    double *e = zeros(m), *ne = zeros(m), *dp = zeros(n);
+    unsigned int seed = 1234;
+    for (int i = 0; i < m; ++i) e[i] = ((double)rand_r(&seed) / (double)RAND_MAX) * 2.0 - 1.0;
+    double s2 = 1.0;
+    double inv = 1.0 / nrm2(m, e);
+    for (int i = 0; i < m; ++i) e[i] *= inv;
+    for (int it = 0; it < maxit; ++it) {
+        spmv(At, e, dp);
+        spmv(A, dp, ne);
+        s2 = dot(m, ne, e);
+        for (int i = 0; i < m; ++i) ne[i] += -s2 * e[i];
+        double res = nrm2(m, ne);
+        if (res < tol) break;
+        for (int i = 0; i < m; ++i) ne[i] += s2 * e[i];
+        inv = 1.0 / nrm2(m, ne);
+        for (int i = 0; i < m; ++i) ne[i] *= inv;
+        double *t = e; e = ne; ne = t;
+    }
+    free(e); free(ne); free(dp);
+    return sqrt(s2);
+}
+
+/* reference src/utils.cu:82-134 */
+static qp_problem_t *with_dummy_row(const qp_problem_t *p) {
+    qp_problem_t *d = (qp_problem_t *)calloc(1, sizeof(qp_problem_t));
+    *d = *p; /* shallow: only the constraint part is replaced */
+    d->num_constraints = 1;
+    d->constraint_matrix_num_nonzeros = 1;
+    d->constraint_matrix = (CsrComponent *)calloc(1, sizeof(CsrComponent));
+    d->constraint_matrix->row_ptr = (int *)xm(2 * sizeof(int));
+    d->constraint_matrix->row_ptr[0] = 0;
+    d->constraint_matrix->row_ptr[1] = 1;
+    d->constraint_matrix->col_ind = (int *)xm(sizeof(int));
+    d->constraint_matrix->col_ind[0] = 0;
+    d->constraint_matrix->val = (double *)xm(sizeof(double));
+    d->constraint_matrix->val[0] = 1.0;
+    d->constraint_lower_bound = (double *)xm(sizeof(double));
+    d->constraint_lower_bound[0] = -INFINITY;
+    d->constraint_upper_bound = (double *)xm(sizeof(double));
+    d->constraint_upper_bound[0] = INFINITY;
+    d->affine_cone_offset = zeros(1);
+    d->dual_start = zeros(1);
+    return d;
+}
+static void free_dummy(qp_problem_t *d) {
+    if (!d) return;
+    free(d->constraint_matrix->row_ptr); free(d->constraint_matrix->col_ind); free(d->constraint_matrix->val);
+    free(d->constraint_matrix);
+    free(d->constraint_lower_bound); free(d->constraint_upper_bound); free(d->affine_cone_offset); free(d->dual_start);
+    free(d);
+}
+
+/* ---- state setup: reference src/solver_state.cu:280-420,688-1058 ---- */
+static void init_inner(orc_solver_t *s) {
+    const pdhg_parameters_t *pp = &s->params;
+    int lim = pp->inner_solver_parameters.iteration_limit;
+    double t0 = pp->inner_solver_parameters.initial_tolerance, tmin = pp->inner_solver_parameters.min_tolerance;
+    if (lim < 1) lim = 1;
+    if (t0 <= 0.0) t0 = 1e-3;
+    if (tmin <= 0.0) tmin = 1e-9;
+    quad_obj_type_t qt = s->w->qtype;
+    s->inner_limit = lim;
+    if (qt == PDHCG_NON_Q || qt == PDHCG_DIAG_Q) return;
+    int n = s->n;
+    s->bb = 1;
+    s->pbuf = zeros(n);
+    s->grad = zeros(n);
+    s->dirn = zeros(n);
+    s->inner_tol = t0;
+    s->inner_min_tol = tmin;
+    s->precond = pp->diag_jacobi_precond;
+    if (s->precond) {
+        s->diag_h = zeros(n);
+        s->mdiag = zeros(n);
+        s->minv = zeros(n);
+        s->cached_inv_tau = -1.0;
+        s->tol_scale = 1.0;
+        if (qt == PDHCG_SPARSE_Q || qt == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
+            for (int i = 0; i < n; ++i) {
+                double acc = 0.0;
+                for (int e = s->w->Q.rp[i]; e < s->w->Q.rp[i + 1]; ++e)
+                    if (s->w->Q.ci[e] == i) acc += s->w->Q.v[e];
+                s->diag_h[i] = acc;
+            }
+        if (qt == PDHCG_LOW_RANK_Q || qt == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+            /* diag(R' D R): rows of R' (reference src/kernels/pdhcg_kernels.cu:425-495) */
+            const orc_csr *T = &s->Rt;
+            for (int i = 0; i < n; ++i) {
+                double acc = 0.0;
+                if (s->w->dkind == 1) {
+                    for (int e = T->rp[i]; e < T->rp[i + 1]; ++e) acc += s->w->ddiag[T->ci[e]] * T->v[e] * T->v[e];
+                } else if (s->w->dkind == 2) {
+                    for (int a = T->rp[i]; a < T->rp[i + 1]; ++a) {
+                        const double *Dr = s->w->ddense + (size_t)T->ci[a] * s->k;
+                        for (int b = T->rp[i]; b < T->rp[i + 1]; ++b) acc += T->v[a] * T->v[b] * Dr[T->ci[b]];
+                    }
+                } else {
+                    for (int e = T->rp[i]; e < T->rp[i + 1]; ++e) acc += T->v[e] * T->v[e];
+                }
+                s->diag_h[i] += acc;
+            }
+        }
+    }
+}
+
+orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params) {
+    if (!prob) return NULL;
+    if (prob->cones.num_cones > 0 || prob->affine_cones.num_cones > 0 || prob->num_quadratic_constraints > 0) {
+        fprintf(stderr, "[oracle] cone and quadratic-constraint problems are not restated in this oracle.\n");
+        return NULL;
+    }
+    orc_solver_t *s = (orc_solver_t *)calloc(1, sizeof(orc_solver_t));
+    if (params) s->params = *params;
+    else orc_set_default_parameters(&s->params);
+    s->orig = prob;
+    const qp_problem_t *work = prob;
+    if (prob->num_constraints == 0 || prob->constraint_matrix == NULL) {
+        s->dummy = with_dummy_row(prob);
+        work = s->dummy;
+    }
+    s->w = orc_rescale(&s->params, work);
+    orc_scaled *w = s->w;
+    int n = w->n, m = w->m;
+    s->n = n; s->m = m; s->k = w->k;
+    csr_transpose(&w->A, &s->At);
+    if (w->R.nnz > 0 || w->qtype == PDHCG_LOW_RANK_Q || w->qtype == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
+        csr_transpose(&w->R, &s->Rt);
+    s->x0 = zeros(n); s->x = zeros(n); s->xp = zeros(n); s->xr = zeros(n);
+    s->aty = zeros(n); s->dslack = zeros(n); s->dres = zeros(n); s->dx = zeros(n);
+    s->y0 = zeros(m); s->y = zeros(m); s->yp = zeros(m); s->yr = zeros(m);
+    s->ax = zeros(m); s->pslack = zeros(m); s->pres = zeros(m); s->dy = zeros(m);
+    s->qx = zeros(n); s->rx = zeros(s->k); s->rxbuf = zeros(s->k);
+    if (work->primal_start)
+        for (int i = 0; i < n; ++i) s->x0[i] = s->x[i] = s->xp[i] = work->primal_start[i] * w->var_rescale[i] * w->cbr;
+    if (work->dual_start)
+        for (int i = 0; i < m; ++i) s->y0[i] = s->y[i] = s->yp[i] = work->dual_start[i] * w->con_rescale[i] * w->ovr;
+    s->cl_fin = zeros(m); s->cu_fin = zeros(m); s->lb_fin = zeros(n); s->ub_fin = zeros(n);
+    for (int i = 0; i < m; ++i) {
+        s->cl_fin[i] = isfinite(w->cl[i]) ? w->cl[i] : 0.0;
+        s->cu_fin[i] = isfinite(w->cu[i]) ? w->cu[i] : 0.0;
+    }
+    for (int j = 0; j < n; ++j) {
+        s->lb_fin[j] = isfinite(w->lb[j]) ? w->lb[j] : 0.0;
+        s->ub_fin[j] = isfinite(w->ub[j]) ? w->ub[j] : 0.0;
+    }
+    /* norms of the unscaled working problem: reference src/solver_state.cu:854-934 */
+    int linf = s->params.optimality_norm == NORM_TYPE_L_INF;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double c = work->objective_vector[j];
+        if (linf) { if (fabs(c) > acc) acc = fabs(c); }
+        else acc += c * c;
+    }
+    s->obj_norm = linf ? acc : sqrt(acc);
+    acc = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double lo = work->constraint_lower_bound[i], hi = work->constraint_upper_bound[i];
+        if (linf) {
+            if (isfinite(lo) && lo != hi && fabs(lo) > acc) acc = fabs(lo);
+            if (isfinite(hi) && fabs(hi) > acc) acc = fabs(hi);
+        } else {
+            if (isfinite(lo) && lo != hi) acc += lo * lo;
+            if (isfinite(hi)) acc += hi * hi;
+        }
+    }
+    for (int i = 0; i < m; ++i) {
+        double g = work->affine_cone_offset[i];
+        if (linf) acc = fmax(acc, fabs(g));
+        else acc += g * g;
+    }
+    s->bound_norm = linf ? acc : sqrt(acc);
+    s->best_gap = INFINITY;
+    s->last_trial_fpe = INFINITY;
+    /* initial projection onto the box: reference src/solver_state.cu:981-995 */
+    for (int j = 0; j < n; ++j) s->x0[j] = fmax(w->lb[j], fmin(s->x0[j], w->ub[j]));
+    memcpy(s->x, s->x0, (size_t)n * sizeof(double));
+    memcpy(s->xp, s->x0, (size_t)n * sizeof(double));
+    memcpy(s->xr, s->x0, (size_t)n * sizeof(double));
+    /* quadratic term information: reference src/solver_state.cu:380-420 */
+    if (w->qtype == PDHCG_SPARSE_Q) {
+        s->q_norm = est_max_eig(&w->Q, s->params.sv_max_iter, s->params.sv_tol);
+        s->nonconvexity = est_min_eig(&w->Q, s->q_norm, s->params.sv_max_iter, s->params.sv_tol);
+    } else if (w->qtype == PDHCG_DIAG_Q) {
+        double mx = 0.0, mn = 0.0;
+        for (int j = 0; j < n; ++j) { mx = fmax(mx, fabs(w->qdiag[j])); mn = fmin(mn, w->qdiag[j]); }
+        s->q_norm = mx;
+        s->nonconvexity = mn;
+    }
+    init_inner(s);
+    s->ptype = w->qtype == PDHCG_NON_Q ? P_LP : P_QP;
+    if (s->nonconvexity < 0) s->inner_limit = 1;
+    /* step size and primal weight: reference src/pdhg_core_op.cu:933-971 */
+    if (w->A.nnz == 0) s->step_size = 1.0;
+    else {
+        double sv = est_max_sv(&w->A, &s->At, s->params.sv_max_iter, s->params.sv_tol);
+        s->step_size = sv < 1e-12 ? 1.0 : 0.998 / sv;
+    }
+    s->primal_weight = s->params.bound_objective_rescaling ? 1.0 : (s->obj_norm + 1.0) / (s->bound_norm + 1.0);
+    s->best_pw = s->primal_weight;
+    clock_gettime(CLOCK_MONOTONIC, &s->t_start);
+    return s;
+}
+
+/* ---- BB inner solve: reference src/pdhg_core_op.cu:495-669 ---- */
+static void bb_prox(orc_solver_t *s, double tau) {
+    int n = s->n;
+    orc_scaled *w = s->w;
+    double inv = 1.0 / tau, a0 = 1.0 / inv;
+    int it = 1;
+    if (s->precond && s->cached_inv_tau != inv) {
+        double sum = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double mm = s->diag_h[i] + inv;
+            if (mm <= 0.0) mm = 1.0;
+            s->mdiag[i] = mm;
+            s->minv[i] = 1.0 / mm;
+            sum += fabs(mm);
+        }
+        s->cached_inv_tau = inv;
+        s->tol_scale = n > 0 ? sqrt(sum / (double)n) : 1.0;
+    }
+    if (s->precond) a0 *= s->tol_scale * s->tol_scale;
+    obj_product(s, s->x);
+    for (int i = 0; i < n; ++i) {
+        double g = s->qx[i] + w->c[i] - s->aty[i];
+        double nx = s->precond ? s->x[i] - a0 * s->minv[i] * g : s->x[i] - a0 * g;
+        nx = clampd(nx, w->lb[i], w->ub[i]);
+        s->xp[i] = nx;
+        s->grad[i] = g;
+        s->dirn[i] = nx - s->x[i];
+    }
+    double stms = 0.0, nrm = 0.0;
+    while (it < s->inner_limit) {
+        if (s->precond) {
+            stms = 0.0;
+            for (int i = 0; i < n; ++i) stms += s->dirn[i] * (s->mdiag[i] * s->dirn[i]);
+            nrm = sqrt(stms);
+        } else {
+            nrm = sqrt(dot(n, s->dirn, s->dirn));
+        }
+        if (nrm <= s->inner_tol) break;
+        obj_product(s, s->xp);
+        for (int i = 0; i < n; ++i) {
+            double last = s->grad[i];
+            double cur = s->qx[i] + w->c[i] - s->aty[i] + inv * (s->xp[i] - s->x[i]);
+            s->pbuf[i] = cur - last;
+            s->grad[i] = cur;
+        }
+        double tmp = dot(n, s->dirn, s->pbuf);
+        double alpha = s->precond ? stms / tmp : (nrm * nrm) / tmp;
+        for (int i = 0; i < n; ++i) {
+            double cur = s->xp[i];
+            double nx = s->precond ? cur - alpha * s->minv[i] * s->grad[i] : cur - alpha * s->grad[i];
+            nx = clampd(nx, w->lb[i], w->ub[i]);
+            s->dirn[i] = nx - cur;
+            s->xp[i] = nx;
+        }
+        it++;
+    }
+    for (int i = 0; i < n; ++i) s->xr[i] = 2.0 * s->xp[i] - s->x[i];
+    s->inner_total += it - 1;
+}
+
+/* ---- one PDHG step: reference src/pdhg_core_op.cu:705-830 ---- */
+static void pdhg_step(orc_solver_t *s) {
+    orc_scaled *w = s->w;
+    int n = s->n, m = s->m;
+    double tau = s->step_size / s->primal_weight;
+    if (s->nonconvexity < 0) {
+        tau = fmax(tau, -1.01 * fmin(0.0, s->nonconvexity));
+        tau /= 100;
+    }
+    double sigma = s->step_size * s->primal_weight;
+    spmv(&s->At, s->y, s->aty);
+    int tc = s->total_count;
+    int store = s->is_major || ((tc + 2) % print_freq(tc + 2)) == 0;
+    switch (w->qtype) {
+    case PDHCG_NON_Q:
+        for (int i = 0; i < n; ++i) {
+            double xi = s->x[i];
+            double t = xi - tau * (w->c[i] - s->aty[i]);
+            double pj = clampd(t, w->lb[i], w->ub[i]);
+            s->xr[i] = 2.0 * pj - xi;
+            if (store) {
+                s->xp[i] = pj;
+                s->dslack[i] = (pj - t) / tau;
+            }
+        }
+        break;
+    case PDHCG_DIAG_Q:
+        for (int i = 0; i < n; ++i) {
+            double xi = s->x[i];
+            double t = (xi - tau * (w->c[i] - s->aty[i])) / (1.0 + tau * w->qdiag[i]);
+            double pj = clampd(t, w->lb[i], w->ub[i]);
+            s->xr[i] = 2.0 * pj - xi;
+            if (store) s->xp[i] = pj;
+        }
+        break;
+    default:
+        bb_prox(s, tau);
+        break;
+    }
+    s->inner_total++;
+    spmv(&w->A, s->xr, s->ax);
+    if (m == 0) return;
+    for (int i = 0; i < m; ++i) {
+        double cur = s->y[i];
+        double v = s->ax[i] + w->g[i];
+        double pj = clampd(v - cur / sigma, w->cl[i], w->cu[i]);
+        double nx = cur - sigma * v + sigma * pj;
+        if (store) s->yp[i] = nx;
+        s->yr[i] = 2.0 * nx - cur;
+    }
+}
+
+/* reference src/pdhg_core_op.cu:832-845 */
+static void halpern(orc_solver_t *s, double rho) {
+    double wgt = (double)(s->inner_count + 1) / (s->inner_count + 2);
+    for (int i = 0; i < s->n; ++i) {
+        double r = rho * s->xr[i] + (1.0 - rho) * s->x[i];
+        s->x[i] = wgt * r + (1.0 - wgt) * s->x0[i];
+    }
+    for (int i = 0; i < s->m; ++i) {
+        double r = rho * s->yr[i] + (1.0 - rho) * s->y[i];
+        s->y[i] = wgt * r + (1.0 - wgt) * s->y0[i];
+    }
+}
+
+/* reference src/pdhg_core_op.cu:973-1035 */
+static void fixed_point_error(orc_solver_t *s) {
+    int n = s->n, m = s->m;
+    for (int i = 0; i < n; ++i) s->dx[i] = s->xr[i] - s->x[i];
+    for (int i = 0; i < m; ++i) s->dy[i] = s->yr[i] - s->y[i];
+    spmv(&s->At, s->dy, s->aty);
+    double dn = nrm2(m, s->dy), pn = nrm2(n, s->dx);
+    dn = sqrt(dn * dn);
+    pn = sqrt(pn * pn);
+    double mov = pn * pn * s->primal_weight + dn * dn / s->primal_weight;
+    double cross = dot(n, s->aty, s->dx);
+    s->fpe = sqrt(mov + 2 * s->step_size * cross);
+    if (s->ptype == P_QP && s->bb)
+        s->inner_tol = fmin(s->inner_tol, fmax(0.0005 * pn / s->step_size * s->primal_weight, s->inner_min_tol));
+}
+
+/* reference src/pdhg_core_op.cu:859-931 */
+static void restart(orc_solver_t *s) {
+    int n = s->n, m = s->m;
+    const restart_parameters_t *rp = &s->params.restart_params;
+    for (int i = 0; i < n; ++i) s->dx[i] = s->xp[i] - s->x0[i];
+    for (int i = 0; i < m; ++i) s->dy[i] = s->yp[i] - s->y0[i];
+    double pd = nrm2(n, s->dx), dd = nrm2(m, s->dy);
+    pd = sqrt(pd * pd);
+    dd = sqrt(dd * dd);
+    double ratio = s->rel_du / s->rel_pr;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        double err = log(dd) - log(pd) - log(s->primal_weight);
+        s->pw_err_sum *= rp->i_smooth;
+        s->pw_err_sum += err;
+        double derr = err - s->pw_last_err;
+        s->primal_weight *= exp(rp->k_p * err + rp->k_i * s->pw_err_sum + rp->k_d * derr);
+        s->pw_last_err = err;
+    } else {
+        s->primal_weight = s->best_pw;
+        s->pw_err_sum = 0.0;
+        s->pw_last_err = 0.0;
+    }
+    double g = fabs(log10(s->rel_du / s->rel_pr));
+    if (g < s->best_gap) {
+        s->best_gap = g;
+        s->best_pw = s->primal_weight;
+    }
+    memcpy(s->x0, s->xp, (size_t)n * sizeof(double));
+    memcpy(s->x, s->xp, (size_t)n * sizeof(double));
+    memcpy(s->y0, s->yp, (size_t)m * sizeof(double));
+    memcpy(s->y, s->yp, (size_t)m * sizeof(double));
+    s->inner_count = 0;
+    s->last_trial_fpe = INFINITY;
+}
+
+/* reference src/utils.cu:291-322 */
+static int should_restart(orc_solver_t *s) {
+    const restart_parameters_t *rp = &s->params.restart_params;
+    int ef = s->params.termination_evaluation_frequency;
+    int r = 0;
+    if (s->total_count == ef) r = 1;
+    else if (s->total_count > ef) {
+        if (s->fpe <= rp->sufficient_reduction_for_restart * s->init_fpe) r = 1;
+        if (s->fpe <= rp->necessary_reduction_for_restart * s->init_fpe && s->fpe > s->last_trial_fpe) r = 1;
+        if (s->inner_count >= rp->artificial_restart_threshold * s->total_count) r = 1;
+    }
+    s->last_trial_fpe = s->fpe;
+    return r;
+}
+
+/* ---- KKT residuals: reference src/pdhg_core_op.cu:1037-1294, kernels :638-752 ---- */
+static void residual(orc_solver_t *s) {
+    orc_scaled *w = s->w;
+    int n = s->n, m = s->m;
+    int linf = s->params.optimality_norm == NORM_TYPE_L_INF;
+    double tau = s->step_size / s->primal_weight;
+    spmv(&w->A, s->xp, s->ax);
+    spmv(&s->At, s->yp, s->aty);
+    obj_product(s, s->xp);
+    for (int i = 0; i < m; ++i) {
+        double v = s->ax[i] + w->g[i];
+        double pj = clampd(v, w->cl[i], w->cu[i]);
+        s->pres[i] = (v - pj) * w->con_rescale[i];
+        s->pslack[i] = fmax(s->yp[i], 0.0) * s->cl_fin[i] + fmin(s->yp[i], 0.0) * s->cu_fin[i] - w->g[i] * s->yp[i];
+    }
+    if (s->ptype == P_LP) {
+        for (int j = 0; j < n; ++j) s->dres[j] = (w->c[j] - s->aty[j] - s->dslack[j]) * w->var_rescale[j];
+    } else {
+        for (int j = 0; j < n; ++j) {
+            double gr = s->qx[j] + w->c[j] - s->aty[j];
+            double t = s->xp[j] - tau * gr;
+            double pj = fmax(w->lb[j], fmin(w->ub[j], t));
+            double ds = (pj - t) / tau;
+            s->dres[j] = (gr - ds) * w->var_rescale[j];
+            s->dslack[j] = ds;
+        }
+    }
+    s->abs_pr = (linf ? inf_norm(m, s->pres) : nrm2(m, s->pres)) / w->cbr;
+    double lin_pr = s->abs_pr;
+    s->abs_du = (linf ? inf_norm(n, s->dres) : nrm2(n, s->dres)) / w->ovr;
+    double half_xqx = w->qtype == PDHCG_NON_Q ? 0.0 : 0.5 * dot(n, s->xp, s->qx);
+    s->pobj = (dot(n, w->c, s->xp) + half_xqx) / (w->cbr * w->ovr) + w->obj_const;
+    double base = dot(n, s->dslack, s->xp);
+    double sps = 0.0;
+    for (int i = 0; i < m; ++i) sps += s->pslack[i];
+    s->dobj = (base + sps - half_xqx) / (w->cbr * w->ovr) + w->obj_const;
+    s->rel_pr = lin_pr / (1.0 + s->bound_norm);
+    double den;
+    if (s->ptype == P_LP) den = 1.0 + s->obj_norm;
+    else {
+        for (int j = 0; j < n; ++j) {
+            s->aty[j] *= w->var_rescale[j];
+            s->qx[j] *= w->var_rescale[j];
+        }
+        double qn = linf ? inf_norm(n, s->qx) : nrm2(n, s->qx);
+        double an = linf ? inf_norm(n, s->aty) : nrm2(n, s->aty);
+        den = 1.0 + fmax(s->obj_norm, fmax(qn / w->ovr, an / w->ovr));
+    }
+    s->rel_du = s->abs_du / den;
+    s->gap = fabs(s->pobj - s->dobj);
+    s->rel_gap = s->gap / (1.0 + fabs(s->pobj) + fabs(s->dobj));
+}
+
+/* reference src/pdhg_core_op.cu:1296-1429 */
+static void infeasibility(orc_solver_t *s) {
+    orc_scaled *w = s->w;
+    int n = s->n, m = s->m;
+    for (int j = 0; j < n; ++j) {
+        if (isfinite(w->lb[j])) s->dx[j] = fmax(s->dx[j], 0.0);
+        if (isfinite(w->ub[j])) s->dx[j] = fmin(s->dx[j], 0.0);
+    }
+    for (int i = 0; i < m; ++i) {
+        if (!isfinite(w->cl[i])) s->dy[i] = fmin(s->dy[i], 0.0);
+        if (!isfinite(w->cu[i])) s->dy[i] = fmax(s->dy[i], 0.0);
+    }
+    double prn = inf_norm(n, s->dx);
+    if (prn > 0.0) {
+        double sc = 1.0 / prn;
+        for (int j = 0; j < n; ++j) s->dx[j] *= sc;
+    }
+    double drn = inf_norm(m, s->dy);
+    spmv(&w->A, s->dx, s->ax);
+    spmv(&s->At, s->dy, s->aty);
+    s->pray_lin_obj = dot(n, w->c, s->dx) / (w->cbr * w->ovr);
+    for (int i = 0; i < m; ++i)
+        s->pslack[i] = fmax(s->dy[i], 0.0) * s->cl_fin[i] + fmin(s->dy[i], 0.0) * s->cu_fin[i] - w->g[i] * s->dy[i];
+    for (int j = 0; j < n; ++j)
+        s->dslack[j] = fmax(-s->aty[j], 0.0) * s->lb_fin[j] + fmin(-s->aty[j], 0.0) * s->ub_fin[j];
+    double sps = 0.0, sds = 0.0;
+    for (int i = 0; i < m; ++i) sps += s->pslack[i];
+    for (int j = 0; j < n; ++j) sds += s->dslack[j];
+    s->dray_obj = (sps + sds) / (w->cbr * w->ovr);
+    for (int i = 0; i < m; ++i) {
+        double pv = s->ax[i];
+        s->pslack[i] = (fmax(0.0, -pv) * (double)isfinite(w->cl[i]) + fmax(0.0, pv) * (double)isfinite(w->cu[i])) *
+                       w->con_rescale[i];
+    }
+    for (int j = 0; j < n; ++j) {
+        double dv = -s->aty[j];
+        s->dslack[j] = (fmax(0.0, dv) * (double)!isfinite(w->lb[j]) - fmin(0.0, dv) * (double)!isfinite(w->ub[j])) *
+                       w->var_rescale[j];
+    }
+    s->max_pray = inf_norm(m, s->pslack);
+    if (s->ptype != P_LP && w->qtype != PDHCG_NON_Q) {
+        obj_product(s, s->dx);
+        double qn = inf_norm(n, s->qx);
+        s->max_pray = fmax(s->max_pray, qn / w->ovr);
+    }
+    double dsn = inf_norm(n, s->dslack);
+    s->max_dray = dsn;
+    double sc = fmax(drn, dsn);
+    if (sc > 0.0) {
+        s->max_dray /= sc;
+        s->dray_obj /= sc;
+    } else {
+        s->max_dray = 0.0;
+        s->dray_obj = 0.0;
+    }
+}
+
+/* reference src/utils.cu:219-289 */
+static void check_termination(orc_solver_t *s) {
+    const termination_criteria_t *c = &s->params.termination_criteria;
+    if (s->rel_du < c->eps_feasible_relative && s->rel_pr < c->eps_feasible_relative &&
+        s->rel_gap < c->eps_optimal_relative) {
+        s->reason = TERMINATION_REASON_OPTIMAL;
+        return;
+    }
+    if (s->dray_obj > 0.0 && s->max_dray / s->dray_obj <= c->eps_infeasible) {
+        s->reason = TERMINATION_REASON_PRIMAL_INFEASIBLE;
+        return;
+    }
+    if (s->pray_lin_obj < 0.0 && s->max_pray / (-s->pray_lin_obj) <= c->eps_infeasible) {
+        s->reason = TERMINATION_REASON_DUAL_INFEASIBLE;
+        return;
+    }
+    if (s->total_count >= c->iteration_limit) {
+        s->reason = TERMINATION_REASON_ITERATION_LIMIT;
+        return;
+    }
+    if (s->cum_time >= c->time_sec_limit) s->reason = TERMINATION_REASON_TIME_LIMIT;
+}
+
+static double elapsed(const orc_solver_t *s) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)(t.tv_sec - s->t_start.tv_sec) + 1e-9 * (double)(t.tv_nsec - s->t_start.tv_nsec);
+}
+
+/* Body of the main loop, reference src/solver.cu:121-169. */
+int orc_solver_run(orc_solver_t *s, int iters) {
+    const pdhg_parameters_t *p = &s->params;
+    int ef = p->termination_evaluation_frequency;
+    for (int done = 0; done < iters; ++done) {
+        if (s->reason != TERMINATION_REASON_UNSPECIFIED) break;
+        if (!(s->total_count < p->termination_criteria.iteration_limit)) {
+            s->reason = TERMINATION_REASON_ITERATION_LIMIT;
+            residual(s);
+            break;
+        }
+        int tc = s->total_count;
+        if (s->is_major || tc == 0 || tc % print_freq(tc) == 0) {
+            residual(s);
+            if (s->is_major && tc < 3 * ef) infeasibility(s);
+            s->cum_time = elapsed(s);
+            check_termination(s);
+            if (s->reason != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        if (s->is_major || tc == 0) {
+            s->do_restart = should_restart(s);
+            if (s->do_restart) restart(s);
+        }
+        s->is_major = ((tc + 1) % ef) == 0;
+        pdhg_step(s);
+        if (s->is_major || s->do_restart) {
+            fixed_point_error(s);
+            if (s->do_restart) {
+                s->init_fpe = s->fpe;
+                s->do_restart = 0;
+            }
+        }
+        halpern(s, p->reflection_coefficient);
+        s->inner_count++;
+        s->total_count++;
+    }
+    return (int)s->reason;
+}
+
+int orc_solver_total_count(const orc_solver_t *s) { return s->total_count; }
+int orc_solver_inner_count(const orc_solver_t *s) { return s->inner_total; }
+
+void orc_solver_scalings(const orc_solver_t *s, double *cr, double *vr, double *cbr, double *ovr, double *step,
+                         double *pw) {
+    if (cr) memcpy(cr, s->w->con_rescale, (size_t)s->m * sizeof(double));
+    if (vr) memcpy(vr, s->w->var_rescale, (size_t)s->n * sizeof(double));
+    if (cbr) *cbr = s->w->cbr;
+    if (ovr) *ovr = s->w->ovr;
+    if (step) *step = s->step_size;
+    if (pw) *pw = s->primal_weight;
+}
+
+/* reference src/pdhg_core_op.cu:1431-1496 */
+static pdhcg_result_t *make_result(orc_solver_t *s) {
+    orc_scaled *w = s->w;
+    int n = s->n, m = s->m;
+    pdhcg_result_t *r = (pdhcg_result_t *)calloc(1, sizeof(pdhcg_result_t));
+    spmv(&s->At, s->yp, s->aty);
+    obj_product(s, s->xp);
+    r->primal_solution = (double *)xm((size_t)n * sizeof(double));
+    r->dual_solution = (double *)xm((size_t)m * sizeof(double));
+    r->reduced_cost = (double *)xm((size_t)n * sizeof(double));
+    for (int j = 0; j < n; ++j) {
+        double gr = w->c[j];
+        if (w->qtype != PDHCG_NON_Q) gr += s->qx[j];
+        double rc = (gr - s->aty[j]) * w->var_rescale[j] / w->ovr;
+        if (!isfinite(w->lb[j])) rc = fmin(rc, 0.0);
+        if (!isfinite(w->ub[j])) rc = fmax(rc, 0.0);
+        r->reduced_cost[j] = rc;
+        r->primal_solution[j] = s->xp[j] / w->var_rescale[j] / w->cbr;
+    }
+    for (int i = 0; i < m; ++i) r->dual_solution[i] = s->yp[i] / w->con_rescale[i] / w->ovr;
+    r->num_variables = s->orig->num_variables;
+    r->num_constraints = s->orig->num_constraints;
+    r->num_nonzeros = s->orig->constraint_matrix_num_nonzeros;
+    r->total_count = s->total_count;
+    r->total_inner_count = s->inner_total;
+    r->rescaling_time_sec = w->rescale_time;
+    r->cumulative_time_sec = s->cum_time;
+    r->relative_primal_residual = s->rel_pr;
+    r->relative_dual_residual = s->rel_du;
+    r->absolute_primal_residual = s->abs_pr;
+    r->absolute_dual_residual = s->abs_du;
+    r->primal_objective_value = s->pobj;
+    r->dual_objective_value = s->dobj;
+    r->objective_gap = s->gap;
+    r->relative_objective_gap = s->rel_gap;
+    r->max_primal_ray_infeasibility = s->max_pray;
+    r->max_dual_ray_infeasibility = s->max_dray;
+    r->primal_ray_linear_objective = s->pray_lin_obj;
+    r->dual_ray_objective = s->dray_obj;
+    r->termination_reason = s->reason;
+    return r;
+}
+
+void orc_solver_free(orc_solver_t *s) {
+    if (!s) return;
+    orc_scaled_free(s->w);
+    orc_csr_free(&s->At);
+    orc_csr_free(&s->Rt);
+    free_dummy(s->dummy);
+    double *arr[] = {s->x0, s->x, s->xp, s->xr, s->y0, s->y, s->yp, s->yr, s->aty, s->ax, s->dslack, s->pslack,
+                     s->pres, s->dres, s->dx, s->dy, s->qx, s->rx, s->rxbuf, s->cl_fin, s->cu_fin, s->lb_fin,
+                     s->ub_fin, s->grad, s->dirn, s->pbuf, s->diag_h, s->mdiag, s->minv};
+    for (size_t i = 0; i < sizeof(arr) / sizeof(arr[0]); ++i) free(arr[i]);
+    free(s);
+}
+
+pdhcg_result_t *orc_solve_qp_problem(const qp_problem_t *prob, const pdhg_parameters_t *params) {
+    if (!prob) return NULL;
+    if (orc_validate_fixed_cone_sections(prob) != 0) return NULL;
+    orc_solver_t *s = orc_solver_create(prob, params);
+    if (!s) return NULL;
+    while (s->reason == TERMINATION_REASON_UNSPECIFIED) {
+        if (!(s->total_count < s->params.termination_criteria.iteration_limit)) {
+            s->reason = TERMINATION_REASON_ITERATION_LIMIT;
+            residual(s);
+            break;
+        }
+        orc_solver_run(s, 1 << 30);
+    }
+    pdhcg_result_t *r = make_result(s);
+    orc_solver_free(s);
+    return r;
+}
+
+void orc_result_free(pdhcg_result_t *r) {
+    if (!r) return;
+    free(r->primal_solution); free(r->dual_solution); free(r->reduced_cost);
+    free(r);
+}

@@ -1,0 +1,111 @@
+"""MI355X implementation of the PDHCG solve path behind the pdhcg.h C interface.
+
+The product is the shared library ``libpdhcg_amd.so`` built from ``csrc/``
+(HIP kernels for gfx950 plus the C interface).  This package locates and loads
+it with ctypes and exposes thin helpers; there is no Python or CPU fallback:
+``lib()`` raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+from . import capi
+from .problem import QPData, build, default_params, solve as _solve
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdhcg_amd.so")
+_lib: Optional[C.CDLL] = None
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+def lib() -> C.CDLL:
+    """Loads libpdhcg_amd.so (built by ``__graft_entry__.build()``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        capi.declare_api(L, "")
+        P = C.POINTER
+        L.pdhcg_solver_create.restype = C.c_void_p
+        L.pdhcg_solver_create.argtypes = [P(capi.qp_problem_t), P(capi.pdhg_parameters_t), C.c_int]
+        L.pdhcg_solver_run.restype = C.c_int
+        L.pdhcg_solver_run.argtypes = [C.c_void_p, C.c_int]
+        L.pdhcg_solver_counts.restype = C.c_int
+        L.pdhcg_solver_counts.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.pdhcg_solver_synchronize.restype = C.c_int
+        L.pdhcg_solver_synchronize.argtypes = [C.c_void_p]
+        L.pdhcg_solver_result.restype = P(capi.pdhcg_result_t)
+        L.pdhcg_solver_result.argtypes = [C.c_void_p]
+        L.pdhcg_solver_free.restype = None
+        L.pdhcg_solver_free.argtypes = [C.c_void_p]
+        L.pdhcg_set_precision.restype = None
+        L.pdhcg_set_precision.argtypes = [C.c_int]
+        L.pdhcg_get_precision.restype = C.c_int
+        L.pdhcg_request_cancel.restype = None
+        L.pdhcg_request_cancel.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def solve(p: QPData, precision: int = PRECISION_FP64, **params) -> Optional[capi.Solution]:
+    """Solves through ``solve_qp_problem`` of the MI355X library."""
+    L = lib()
+    L.pdhcg_set_precision(precision)
+    return _solve(L, "", p, default_params(L, "", **params))
+
+
+class Stepper:
+    """Owns a pdhcg_solver_t: create once, run the main loop in chunks."""
+
+    def __init__(self, p: QPData, precision: int = PRECISION_FP64, **params):
+        L = lib()
+        self.L = L
+        self.handle = build(L, "", p)
+        if self.handle is None:
+            raise ValueError("create_qp_problem rejected the problem")
+        self.params = default_params(L, "", **params)
+        self.s = L.pdhcg_solver_create(self.handle.ptr, C.byref(self.params), precision)
+        if not self.s:
+            raise RuntimeError("pdhcg_solver_create failed")
+
+    def run(self, iterations: int) -> int:
+        rc = self.L.pdhcg_solver_run(self.s, iterations)
+        if rc < 0:
+            raise RuntimeError("device iteration failed")
+        return rc
+
+    def synchronize(self):
+        if self.L.pdhcg_solver_synchronize(self.s) != 0:
+            raise RuntimeError("device synchronisation failed")
+
+    def counts(self):
+        t, i = C.c_int(), C.c_int()
+        self.L.pdhcg_solver_counts(self.s, C.byref(t), C.byref(i))
+        return t.value, i.value
+
+    def result(self) -> capi.Solution:
+        r = self.L.pdhcg_solver_result(self.s)
+        if not r:
+            raise RuntimeError("result extraction failed")
+        h = self.handle.contents
+        sol = capi.result_to_solution(r, h.num_variables, h.num_constraints)
+        self.L.pdhcg_result_free(r)
+        return sol
+
+    def close(self):
+        if self.s:
+            self.L.pdhcg_solver_free(self.s)
+            self.s = None
+        if self.handle is not None:
+            self.handle.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

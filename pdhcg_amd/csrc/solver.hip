@@ -1,0 +1,1018 @@
+// Restarted Halpern PDHG with the projected Barzilai-Borwein inner solve on
+// one MI355X (gfx950).  The host drives the outer loop exactly as reference
+// src/solver.cu:121-169 does (evaluation cadence, restarts, fixed-point error,
+// Halpern weights), while every vector operation runs in the HIP kernels of
+// dev_spmv.hpp / dev_kernels.hpp.  Host synchronisation happens only where the
+// host must decide: KKT evaluations, restarts, fixed-point errors, and
+// between chunks of the device-resident BB and power iterations.
+//
+// Template parameters: V is the storage type of matrix values, X of vectors.
+// Solver<double, double> is the plain FP64 form; Solver<float, float> keeps
+// matrix values and vector traffic in FP32 (FP32 SpMV accumulation) with all
+// reductions, scalars, restart and KKT logic in FP64.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pdhcg_amd.h"
+#include "dev_kernels.hpp"
+#include "host_scale.h"
+
+namespace pdhcg {
+
+// Cancellation request checked at every KKT evaluation (reference src/pdhcg.c:31,
+// src/utils.cu:252-256).
+volatile int g_pdhcg_cancel = 0;
+
+namespace {
+
+int print_frequency(int iter) {  // reference src/utils.cu:535-546
+    int step = 10;
+    long long threshold = 1000;
+    while (iter >= threshold) {
+        step *= 10;
+        threshold *= 10;
+    }
+    return step;
+}
+
+template <class T> T *dalloc(size_t count) {
+    T *p = nullptr;
+    HIP_OK(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+    return p;
+}
+
+template <class T, class S> T *dupload(const std::vector<S> &h, size_t count) {
+    T *p = dalloc<T>(count);
+    if (count) {
+        std::vector<T> tmp(count);
+        for (size_t i = 0; i < count; ++i) tmp[i] = (T)h[i];
+        HIP_OK(hipMemcpy(p, tmp.data(), count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    return p;
+}
+
+template <class T> T *dzeros(size_t count) {
+    T *p = dalloc<T>(count);
+    HIP_OK(hipMemset(p, 0, (count ? count : 1) * sizeof(T)));
+    return p;
+}
+
+template <class V> DevCsr<V> upload_csr(const HostCSR &h) {
+    DevCsr<V> d;
+    d.rows = h.rows;
+    d.cols = h.cols;
+    d.nnz = h.nnz();
+    d.rp = dupload<int>(h.rp, h.rp.size());
+    d.ci = dupload<int>(h.ci, h.ci.size());
+    d.v = dupload<V>(h.v, h.v.size());
+    return d;
+}
+
+template <class V> void free_csr(DevCsr<V> &d) {
+    if (d.rp) (void)hipFree(d.rp);
+    if (d.ci) (void)hipFree(d.ci);
+    if (d.v) (void)hipFree(d.v);
+    d = DevCsr<V>();
+}
+
+struct RunStats {
+    int total_count = 0;
+    int inner_total = 0;
+};
+
+}  // namespace
+
+class SolverBase {
+  public:
+    virtual ~SolverBase() = default;
+    virtual int run(int iterations) = 0;
+    virtual pdhcg_result_t *result(const qp_problem_t *original) = 0;
+    virtual void counts(int *total, int *inner) = 0;
+    virtual void synchronize() = 0;
+    virtual double setup_seconds() const = 0;
+};
+
+template <class V, class X> class Solver final : public SolverBase {
+  public:
+    Solver(const ScaledProblem &sp, const pdhg_parameters_t &params);
+    ~Solver() override;
+    int run(int iterations) override;
+    pdhcg_result_t *result(const qp_problem_t *original) override;
+    void counts(int *total, int *inner) override {
+        read_scalars();
+        *total = total_count_;
+        *inner = host_inner_ + hs_->inner_total;
+    }
+    void synchronize() override { HIP_OK(hipStreamSynchronize(st_)); }
+    double setup_seconds() const override { return setup_seconds_; }
+
+  private:
+    using Acc = typename std::conditional<std::is_same<V, double>::value || std::is_same<X, double>::value, double,
+                                          float>::type;
+
+    // ---- setup helpers
+    void upload(const ScaledProblem &sp);
+    void init_inner(const ScaledProblem &sp);
+    double power_sv();
+    double power_eig(bool minimum, double shift);
+
+    // ---- device operations
+    GridRed red() const { return GridRed{red_partials_, red_counter_, ds_->red}; }
+    void read_scalars() {
+        HIP_OK(hipMemcpyAsync(hs_, ds_, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+    int grid(long long n) const { return blocks_for(n); }
+    void q_product(const X *x, X *qx, const int *gate);
+    void bb_prox(double tau, bool fuse_halpern, double w);
+    void pdhg_step(bool fuse_halpern);
+    void halpern(double w);
+    void fixed_point_error();
+    void restart();
+    bool should_restart();
+    void residual();
+    void infeasibility();
+    void check_termination();
+
+    pdhg_parameters_t P_;
+    hipStream_t st_ = nullptr;
+    int n_ = 0, m_ = 0, k_ = 0;
+    quad_obj_type_t qtype_ = PDHCG_NON_Q;
+    bool is_lp_ = true;
+    double cbr_ = 1, ovr_ = 1, obj_const_ = 0, obj_norm_ = 0, bound_norm_ = 0, rescale_time_ = 0;
+    double setup_seconds_ = 0;
+
+    DevCsr<V> A_, At_, Q_, Rc_, Rtc_;
+    SpmvPlan pA_, pAt_, pQ_, pR_, pRt_;
+    bool q_sparse_ = false;          // Q stored as CSR (has off-diagonal entries)
+    X *qsdiag_ = nullptr;            // Q stored as its diagonal
+    X *qdiag_ = nullptr;             // DIAG_Q prox diagonal (diag + 1e-12)
+    V *Rd_ = nullptr;                // dense R, k x n row-major
+    bool r_dense_ = false;
+    int dkind_ = 0;
+    double *ddiag_ = nullptr, *ddense_ = nullptr, *u_ = nullptr, *rx_partials_ = nullptr;
+    unsigned *rx_counter_ = nullptr;
+    X *rxv_ = nullptr;
+    int rx_blocks_ = 0;
+
+    X *lb_ = nullptr, *ub_ = nullptr, *c_ = nullptr, *cl_ = nullptr, *cu_ = nullptr, *g_ = nullptr;
+    X *cl_fin_ = nullptr, *cu_fin_ = nullptr, *lb_fin_ = nullptr, *ub_fin_ = nullptr;
+    double *con_rescale_ = nullptr, *var_rescale_ = nullptr;
+    X *x0_ = nullptr, *x_ = nullptr, *xp_ = nullptr, *xr_ = nullptr;
+    X *y0_ = nullptr, *y_ = nullptr, *yp_ = nullptr, *yr_ = nullptr;
+    X *aty_ = nullptr, *ax_ = nullptr, *dslack_ = nullptr, *dx_ = nullptr, *dy_ = nullptr, *qx_ = nullptr;
+
+    // BB
+    bool bb_ = false, precond_ = false;
+    X *grad_ = nullptr, *dirn_ = nullptr, *mdiag_ = nullptr, *minv_ = nullptr;
+    double *diag_h_ = nullptr;
+    double cached_inv_tau_ = -1.0, inner_tol_ = 1e-3, inner_min_tol_ = 1e-9;
+    int inner_limit_ = 1000;
+    int host_inner_ = 0;  // +1 per outer iteration (reference pdhg_core_op.cu:762)
+
+    // reductions and device scalars
+    double *red_partials_ = nullptr;
+    unsigned *red_counter_ = nullptr;
+    DevScalars *ds_ = nullptr, *hs_ = nullptr;
+
+    // outer-loop state (reference internal/internal_types.h:137-236)
+    double q_norm_ = 0, nonconvexity_ = 0;
+    double step_ = 0, pw_ = 1, pw_err_sum_ = 0, pw_last_err_ = 0, best_pw_ = 1, best_gap_ = INFINITY;
+    double fpe_ = 0, init_fpe_ = 0, last_trial_fpe_ = INFINITY;
+    int total_count_ = 0, inner_count_ = 0;
+    bool is_major_ = false, do_restart_ = false;
+    double abs_pr_ = 0, rel_pr_ = 0, abs_du_ = 0, rel_du_ = 0, pobj_ = 0, dobj_ = 0, gap_ = 0, rel_gap_ = 0;
+    double max_pray_ = 0, max_dray_ = 0, pray_lin_obj_ = 0, dray_obj_ = 0;
+    termination_reason_t reason_ = TERMINATION_REASON_UNSPECIFIED;
+    double cum_time_ = 0;
+    std::chrono::steady_clock::time_point t_start_;
+};
+
+template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params) : P_(params) {
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    n_ = sp.n;
+    m_ = sp.m;
+    k_ = sp.k;
+    qtype_ = sp.qtype;
+    is_lp_ = qtype_ == PDHCG_NON_Q;
+    cbr_ = sp.cbr;
+    ovr_ = sp.ovr;
+    obj_const_ = sp.obj_const;
+    obj_norm_ = sp.obj_norm;
+    bound_norm_ = sp.bound_norm;
+    rescale_time_ = sp.rescale_time;
+    red_partials_ = dzeros<double>((size_t)kMaxRedBlocks * kMaxRedValues);
+    red_counter_ = dzeros<unsigned>(1);
+    ds_ = dzeros<DevScalars>(1);
+    HIP_OK(hipHostMalloc((void **)&hs_, sizeof(DevScalars), hipHostMallocDefault));
+    std::memset(hs_, 0, sizeof(DevScalars));
+    upload(sp);
+
+    // quadratic term information (reference src/solver_state.cu:380-420)
+    if (qtype_ == PDHCG_SPARSE_Q) {
+        q_norm_ = power_eig(false, 0.0);
+        nonconvexity_ = power_eig(true, q_norm_);
+    } else if (qtype_ == PDHCG_DIAG_Q) {
+        double mx = 0.0, mn = 0.0;
+        for (double q : sp.qdiag) {
+            mx = std::fmax(mx, std::fabs(q));
+            mn = std::fmin(mn, q);
+        }
+        q_norm_ = mx;
+        nonconvexity_ = mn;
+    }
+    init_inner(sp);
+    if (nonconvexity_ < 0) inner_limit_ = 1;
+    // step size and primal weight (reference src/pdhg_core_op.cu:933-971)
+    if (A_.nnz == 0) step_ = 1.0;
+    else {
+        const double sv = power_sv();
+        step_ = sv < 1e-12 ? 1.0 : 0.998 / sv;
+    }
+    pw_ = P_.bound_objective_rescaling ? 1.0 : (obj_norm_ + 1.0) / (bound_norm_ + 1.0);
+    best_pw_ = pw_;
+    HIP_OK(hipStreamSynchronize(st_));
+    t_start_ = std::chrono::steady_clock::now();
+    setup_seconds_ = std::chrono::duration<double>(t_start_ - t0).count();
+}
+
+template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
+    A_ = upload_csr<V>(sp.A);
+    pA_ = make_spmv_plan(m_, sp.A.rp);
+    {
+        const HostCSR t = transpose(sp.A);
+        At_ = upload_csr<V>(t);
+        pAt_ = make_spmv_plan(n_, t.rp);
+    }
+    if (qtype_ == PDHCG_SPARSE_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+        bool diag_only = true;
+        for (int i = 0; i < n_ && diag_only; ++i)
+            for (int e = sp.Q.rp[i]; e < sp.Q.rp[(size_t)i + 1]; ++e)
+                if (sp.Q.ci[e] != i) {
+                    diag_only = false;
+                    break;
+                }
+        if (diag_only) {
+            std::vector<double> d((size_t)n_, 0.0);
+            for (int i = 0; i < n_; ++i)
+                for (int e = sp.Q.rp[i]; e < sp.Q.rp[(size_t)i + 1]; ++e) d[i] += sp.Q.v[e];
+            qsdiag_ = dupload<X>(d, (size_t)n_);
+        } else {
+            q_sparse_ = true;
+            Q_ = upload_csr<V>(sp.Q);
+            pQ_ = make_spmv_plan(n_, sp.Q.rp);
+        }
+    }
+    if (qtype_ == PDHCG_DIAG_Q) qdiag_ = dupload<X>(sp.qdiag, (size_t)n_);
+    if (qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+        if (k_ > kRxMaxRank) fail("low-rank term of rank %d exceeds the supported maximum %d", k_, kRxMaxRank);
+        const long long dense_elems = (long long)k_ * n_;
+        r_dense_ = dense_elems <= 4LL * sp.R.nnz() + 4096;
+        if (r_dense_) {
+            std::vector<double> Rd((size_t)dense_elems, 0.0);
+            for (int j = 0; j < k_; ++j)
+                for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e) Rd[(size_t)j * n_ + sp.R.ci[e]] += sp.R.v[e];
+            Rd_ = dupload<V>(Rd, Rd.size());
+            rx_blocks_ = (int)std::max<long long>(1, ((long long)n_ + kBlock * kRxColsPerThread - 1) /
+                                                         (kBlock * kRxColsPerThread));
+            rx_partials_ = dzeros<double>((size_t)rx_blocks_ * k_);
+            rx_counter_ = dzeros<unsigned>(1);
+        } else {
+            Rc_ = upload_csr<V>(sp.R);
+            HostCSR rt = transpose(sp.R);
+            Rtc_ = upload_csr<V>(rt);
+            pR_ = make_spmv_plan(k_, sp.R.rp);
+            pRt_ = make_spmv_plan(n_, rt.rp);
+            rxv_ = dzeros<X>((size_t)k_);
+        }
+        dkind_ = (int)sp.dkind;
+        if (sp.dkind == MiddleKind::Diag) ddiag_ = dupload<double>(sp.ddiag, sp.ddiag.size());
+        if (sp.dkind == MiddleKind::Dense) ddense_ = dupload<double>(sp.ddense, sp.ddense.size());
+        u_ = dzeros<double>((size_t)k_);
+    }
+    lb_ = dupload<X>(sp.lb, n_);
+    ub_ = dupload<X>(sp.ub, n_);
+    c_ = dupload<X>(sp.c, n_);
+    cl_ = dupload<X>(sp.cl, m_);
+    cu_ = dupload<X>(sp.cu, m_);
+    g_ = dupload<X>(sp.g, m_);
+    auto finite_or_zero = [](const std::vector<double> &v) {
+        std::vector<double> r(v.size());
+        for (size_t i = 0; i < v.size(); ++i) r[i] = std::isfinite(v[i]) ? v[i] : 0.0;
+        return r;
+    };
+    cl_fin_ = dupload<X>(finite_or_zero(sp.cl), m_);
+    cu_fin_ = dupload<X>(finite_or_zero(sp.cu), m_);
+    lb_fin_ = dupload<X>(finite_or_zero(sp.lb), n_);
+    ub_fin_ = dupload<X>(finite_or_zero(sp.ub), n_);
+    con_rescale_ = dupload<double>(sp.con_rescale, m_);
+    var_rescale_ = dupload<double>(sp.var_rescale, n_);
+
+    // starting point (reference src/solver_state.cu:809-829,976-995)
+    std::vector<double> xs((size_t)n_, 0.0), ys((size_t)m_, 0.0);
+    if (!sp.primal_start.empty())
+        for (int i = 0; i < n_; ++i) xs[i] = sp.primal_start[i] * sp.var_rescale[i] * sp.cbr;
+    if (!sp.dual_start.empty())
+        for (int i = 0; i < m_; ++i) ys[i] = sp.dual_start[i] * sp.con_rescale[i] * sp.ovr;
+    for (int i = 0; i < n_; ++i) xs[i] = std::fmax(sp.lb[i], std::fmin(xs[i], sp.ub[i]));
+    x0_ = dupload<X>(xs, n_);
+    x_ = dupload<X>(xs, n_);
+    xp_ = dupload<X>(xs, n_);
+    xr_ = dupload<X>(xs, n_);
+    y0_ = dupload<X>(ys, m_);
+    y_ = dupload<X>(ys, m_);
+    yp_ = dupload<X>(ys, m_);
+    yr_ = dzeros<X>(m_);
+    aty_ = dzeros<X>(n_);
+    ax_ = dzeros<X>(m_);
+    dslack_ = dzeros<X>(n_);
+    dx_ = dzeros<X>(n_);
+    dy_ = dzeros<X>(m_);
+    qx_ = dzeros<X>(n_);
+}
+
+// Jacobi metric diag(Q) + diag(R'DR) from the scaled host data
+// (reference src/solver_state.cu:280-370, kernels pdhcg_kernels.cu:407-495).
+template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &sp) {
+    int lim = P_.inner_solver_parameters.iteration_limit;
+    double t0 = P_.inner_solver_parameters.initial_tolerance, tmin = P_.inner_solver_parameters.min_tolerance;
+    if (lim < 1) {
+        std::fprintf(stderr, "Warning: inner_iter_limit (%d) < 1. Resetting to 1.\n", lim);
+        lim = 1;
+    }
+    if (t0 <= 0.0) t0 = 1e-3;
+    if (tmin <= 0.0) tmin = 1e-9;
+    inner_limit_ = lim;
+    if (qtype_ == PDHCG_NON_Q || qtype_ == PDHCG_DIAG_Q) return;
+    bb_ = true;
+    inner_tol_ = t0;
+    inner_min_tol_ = tmin;
+    grad_ = dzeros<X>(n_);
+    dirn_ = dzeros<X>(n_);
+    precond_ = P_.diag_jacobi_precond;
+    if (!precond_) return;
+    std::vector<double> dh((size_t)n_, 0.0);
+    if (qtype_ == PDHCG_SPARSE_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
+        for (int i = 0; i < n_; ++i)
+            for (int e = sp.Q.rp[i]; e < sp.Q.rp[(size_t)i + 1]; ++e)
+                if (sp.Q.ci[e] == i) dh[i] += sp.Q.v[e];
+    if (qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+        const HostCSR rt = transpose(sp.R);
+        for (int i = 0; i < n_; ++i) {
+            double acc = 0.0;
+            const int e0 = rt.rp[i], e1 = rt.rp[(size_t)i + 1];
+            if (sp.dkind == MiddleKind::Diag) {
+                for (int e = e0; e < e1; ++e) acc += sp.ddiag[rt.ci[e]] * rt.v[e] * rt.v[e];
+            } else if (sp.dkind == MiddleKind::Dense) {
+                for (int a = e0; a < e1; ++a) {
+                    const double *Drow = sp.ddense.data() + (size_t)rt.ci[a] * k_;
+                    for (int b = e0; b < e1; ++b) acc += rt.v[a] * rt.v[b] * Drow[rt.ci[b]];
+                }
+            } else {
+                for (int e = e0; e < e1; ++e) acc += rt.v[e] * rt.v[e];
+            }
+            dh[i] += acc;
+        }
+    }
+    diag_h_ = dupload<double>(dh, n_);
+    mdiag_ = dzeros<X>(n_);
+    minv_ = dzeros<X>(n_);
+}
+
+template <class V, class X> Solver<V, X>::~Solver() {
+    if (st_) (void)hipStreamSynchronize(st_);
+    free_csr(A_);
+    free_csr(At_);
+    free_csr(Q_);
+    free_csr(Rc_);
+    free_csr(Rtc_);
+    free_spmv_plan(pA_);
+    free_spmv_plan(pAt_);
+    free_spmv_plan(pQ_);
+    free_spmv_plan(pR_);
+    free_spmv_plan(pRt_);
+    void *ptrs[] = {qsdiag_, qdiag_, Rd_,     ddiag_,  ddense_,      u_,          rx_partials_, rx_counter_,
+                    rxv_,    lb_,    ub_,     c_,      cl_,          cu_,         g_,           cl_fin_,
+                    cu_fin_, lb_fin_, ub_fin_, con_rescale_, var_rescale_, x0_,     x_,           xp_,
+                    xr_,     y0_,    y_,      yp_,     yr_,          aty_,        ax_,          dslack_,
+                    dx_,     dy_,    qx_,     grad_,   dirn_,        mdiag_,      minv_,        diag_h_,
+                    red_partials_, red_counter_, ds_};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (hs_) (void)hipHostFree(hs_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+// ||A||_2 by power iteration on A A' (reference src/pdhg_core_op.cu:1673-1788).
+template <class V, class X> double Solver<V, X>::power_sv() {
+    std::vector<double> e((size_t)m_);
+    unsigned int seed = 1234;
+    for (int i = 0; i < m_; ++i) e[i] = ((double)rand_r(&seed) / (double)RAND_MAX) * 2.0 - 1.0;
+    double nrm = 0.0;
+    for (double v : e) nrm += v * v;
+    nrm = std::sqrt(nrm);
+    const double inv = 1.0 / std::sqrt(nrm * nrm);
+    for (double &v : e) v *= inv;
+    X *buf[2] = {dupload<X>(e, m_), dzeros<X>(m_)};
+    X *dp = dzeros<X>(n_);
+    std::memset(hs_, 0, sizeof(DevScalars));
+    hs_->pi_active = 1;
+    hs_->pi_value = 1.0;
+    HIP_OK(hipMemcpyAsync(ds_, hs_, sizeof(DevScalars), hipMemcpyHostToDevice, st_));
+    const int *gate = &ds_->pi_active;
+    int it = 0, chunk = 8;
+    while (it < P_.sv_max_iter) {
+        const int todo = std::min(chunk, P_.sv_max_iter - it);
+        for (int c = 0; c < todo; ++c, ++it) {
+            X *ev = buf[it % 2], *ne = buf[(it + 1) % 2];
+            spmv_launch<V, X, Acc>(At_, pAt_, ev, EpiStore<X>{dp}, st_, gate);
+            spmv_launch<V, X, Acc>(A_, pA_, dp, EpiStore<X>{ne}, st_, gate);
+            hipLaunchKernelGGL(pi_dot_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, red(), ds_);
+            hipLaunchKernelGGL(pi_residual_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, P_.sv_tol,
+                               red(), ds_);
+            hipLaunchKernelGGL(pi_restore_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, red(), ds_);
+            hipLaunchKernelGGL(scale_dev_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, &ds_->pi_inv, m_, gate);
+        }
+        HIP_OK(hipGetLastError());
+        read_scalars();
+        if (!hs_->pi_active) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    read_scalars();
+    const double s2 = hs_->pi_value;
+    (void)hipFree(buf[0]);
+    (void)hipFree(buf[1]);
+    (void)hipFree(dp);
+    return std::sqrt(s2);
+}
+
+// Largest eigenvalue (minimum = false) or lambda_max - largest eigenvalue of
+// (lambda_max I - Q) (minimum = true) of the sparse Q
+// (reference src/pdhg_core_op.cu:1498-1671).
+template <class V, class X> double Solver<V, X>::power_eig(bool minimum, double shift) {
+    std::vector<double> v0((size_t)n_);
+    unsigned int seed = 1234;
+    for (int i = 0; i < n_; ++i) v0[i] = (double)rand_r(&seed) / RAND_MAX;
+    X *v = dupload<X>(v0, n_), *av = dzeros<X>(n_);
+    std::memset(hs_, 0, sizeof(DevScalars));
+    hs_->pi_active = 1;
+    HIP_OK(hipMemcpyAsync(ds_, hs_, sizeof(DevScalars), hipMemcpyHostToDevice, st_));
+    const int *gate = &ds_->pi_active;
+    int it = 0, chunk = 8;
+    while (it < P_.sv_max_iter) {
+        const int todo = std::min(chunk, P_.sv_max_iter - it);
+        for (int c = 0; c < todo; ++c, ++it) {
+            hipLaunchKernelGGL(pi_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, n_, red(), ds_);
+            hipLaunchKernelGGL(scale_dev_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, &ds_->pi_inv, n_, gate);
+            spmv_launch<V, X, Acc>(Q_, pQ_, v, EpiStore<X>{av}, st_, gate);
+            hipLaunchKernelGGL(pi_rayleigh_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, av, shift,
+                               minimum ? 1 : 0, n_, P_.sv_tol, red(), ds_);
+            hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, av, n_, gate);
+        }
+        HIP_OK(hipGetLastError());
+        read_scalars();
+        if (!hs_->pi_active) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    read_scalars();
+    const double val = hs_->pi_value;
+    (void)hipFree(v);
+    (void)hipFree(av);
+    return minimum ? shift - val : val;
+}
+
+// Q x by objective type (reference src/pdhg_core_op.cu:305-413).
+template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, const int *gate) {
+    const int nb = grid(n_);
+    switch (qtype_) {
+    case PDHCG_NON_Q:
+        return;
+    case PDHCG_DIAG_Q:
+        hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qdiag_, x, qx, n_, gate);
+        return;
+    case PDHCG_SPARSE_Q:
+        if (q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, x, EpiStore<X>{qx}, st_, gate);
+        else hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qsdiag_, x, qx, n_, gate);
+        return;
+    case PDHCG_LOW_RANK_Q:
+    case PDHCG_LOW_RANK_PLUS_SPARSE_Q: {
+        const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
+        if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, x, EpiStore<X>{qx}, st_, gate);
+        if (r_dense_) {
+            hipLaunchKernelGGL((rx_dense_kernel<V, X>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_, k_,
+                               rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+            const X *qd = (plus && !q_sparse_) ? qsdiag_ : nullptr;
+            hipLaunchKernelGGL((rtu_dense_kernel<V, X>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_, qx, qd, x,
+                               (plus && q_sparse_) ? 1 : 0, gate);
+        } else {
+            spmv_launch<V, X, Acc>(Rc_, pR_, x, EpiStore<X>{rxv_}, st_, gate);
+            hipLaunchKernelGGL(apply_middle_kernel<X>, dim3(1), dim3(kBlock), 0, st_, rxv_, u_, k_, ddiag_, ddense_,
+                               dkind_, gate);
+            if (plus && !q_sparse_)
+                hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qsdiag_, x, qx, n_, gate);
+            if (plus) spmv_launch<V, double, double>(Rtc_, pRt_, u_, EpiAccumulate<X>{qx}, st_, gate);
+            else spmv_launch<V, double, double>(Rtc_, pRt_, u_, EpiStore<X>{qx}, st_, gate);
+        }
+        return;
+    }
+    default:
+        fail("unknown quadratic objective type");
+    }
+}
+
+// Projected BB inner solve of the primal prox (reference src/pdhg_core_op.cu:495-669).
+template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_halpern, double w) {
+    const double inv = 1.0 / tau;
+    const double alpha0 = 1.0 / inv;
+    const int nb = grid(n_);
+    if (precond_ && cached_inv_tau_ != inv) {
+        hipLaunchKernelGGL(bb_refresh_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, diag_h_, inv, mdiag_, minv_, n_,
+                           red(), ds_);
+        cached_inv_tau_ = inv;
+    }
+    const X *mi = precond_ ? minv_ : nullptr;
+    const X *md = precond_ ? mdiag_ : nullptr;
+    q_product(x_, qx_, nullptr);
+    hipLaunchKernelGGL(bb_init_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, aty_, grad_, dirn_, x_, xp_, c_, qx_, lb_,
+                       ub_, mi, md, alpha0, n_, red(), ds_, inner_tol_, inner_limit_);
+    const int *gate = &ds_->bb_active;
+    const int max_bodies = inner_limit_ - 1;
+    int issued = 0, chunk = 2;
+    while (issued < max_bodies) {
+        const int todo = std::min(chunk, max_bodies - issued);
+        for (int c = 0; c < todo; ++c) {
+            q_product(xp_, qx_, gate);
+            hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_, grad_,
+                               dirn_, inv, n_, precond_ ? 1 : 0, red(), ds_);
+            hipLaunchKernelGGL(bb_direction_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, grad_, dirn_, lb_, ub_,
+                               mi, md, n_, red(), ds_, inner_tol_, inner_limit_);
+        }
+        issued += todo;
+        HIP_OK(hipGetLastError());
+        if (issued >= max_bodies) break;
+        HIP_OK(hipMemcpyAsync(&hs_->bb_active, &ds_->bb_active, sizeof(int), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        if (!hs_->bb_active) break;
+        chunk = std::min(chunk * 2, 16);
+    }
+    hipLaunchKernelGGL(bb_final_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, x_, xp_, xr_, x0_, n_,
+                       fuse_halpern ? 1 : 0, w, P_.reflection_coefficient, ds_);
+}
+
+// One PDHG step (reference src/pdhg_core_op.cu:705-830).
+template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
+    double tau = step_ / pw_;
+    if (nonconvexity_ < 0) {
+        tau = std::fmax(tau, -1.01 * std::fmin(0.0, nonconvexity_));
+        tau /= 100;
+    }
+    const double sigma = step_ * pw_;
+    const int tc = total_count_;
+    const int store = (is_major_ || ((tc + 2) % print_frequency(tc + 2)) == 0) ? 1 : 0;
+    const double w = (double)(inner_count_ + 1) / (inner_count_ + 2);
+    const double rho = P_.reflection_coefficient;
+    if (qtype_ == PDHCG_NON_Q || qtype_ == PDHCG_DIAG_Q) {
+        EpiPrimalStep<X> epi{x_, xp_, xr_, dslack_, c_, lb_, ub_, x0_, qtype_ == PDHCG_DIAG_Q ? qdiag_ : nullptr,
+                             tau, w, rho, store, fuse_halpern ? 1 : 0};
+        spmv_launch<V, X, Acc>(At_, pAt_, y_, epi, st_);
+    } else {
+        spmv_launch<V, X, Acc>(At_, pAt_, y_, EpiStore<X>{aty_}, st_);
+        bb_prox(tau, fuse_halpern, w);
+    }
+    host_inner_ += 1;
+    EpiDualStep<X> depi{y_, yp_, yr_, g_, cl_, cu_, y0_, sigma, w, rho, store, fuse_halpern ? 1 : 0};
+    spmv_launch<V, X, Acc>(A_, pA_, xr_, depi, st_);
+}
+
+template <class V, class X> void Solver<V, X>::halpern(double w) {
+    hipLaunchKernelGGL(halpern_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, x0_, x_, xr_, y0_, y_,
+                       yr_, n_, m_, w, P_.reflection_coefficient);
+}
+
+// reference src/pdhg_core_op.cu:973-1035
+template <class V, class X> void Solver<V, X>::fixed_point_error() {
+    hipLaunchKernelGGL(delta_norms_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xr_, x_, dx_, yr_,
+                       y_, dy_, n_, m_, red());
+    read_scalars();
+    const double pn2 = hs_->red[0], dn2 = hs_->red[1];
+    spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
+    hipLaunchKernelGGL(dot_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, aty_, dx_, n_, red(), nullptr);
+    read_scalars();
+    const double cross = hs_->red[0];
+    const double pn = std::sqrt(pn2), dn = std::sqrt(dn2);
+    const double movement = pn * pn * pw_ + dn * dn / pw_;
+    fpe_ = std::sqrt(movement + 2 * step_ * cross);
+    if (!is_lp_ && bb_) inner_tol_ = std::fmin(inner_tol_, std::fmax(0.0005 * pn / step_ * pw_, inner_min_tol_));
+}
+
+// reference src/pdhg_core_op.cu:859-931
+template <class V, class X> void Solver<V, X>::restart() {
+    const restart_parameters_t &rp = P_.restart_params;
+    hipLaunchKernelGGL(delta_norms_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, x0_, dx_, yp_,
+                       y0_, dy_, n_, m_, red());
+    read_scalars();
+    const double pd = std::sqrt(hs_->red[0]), dd = std::sqrt(hs_->red[1]);
+    const double ratio = rel_du_ / rel_pr_;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = std::log(dd) - std::log(pd) - std::log(pw_);
+        pw_err_sum_ *= rp.i_smooth;
+        pw_err_sum_ += err;
+        const double derr = err - pw_last_err_;
+        pw_ *= std::exp(rp.k_p * err + rp.k_i * pw_err_sum_ + rp.k_d * derr);
+        pw_last_err_ = err;
+    } else {
+        pw_ = best_pw_;
+        pw_err_sum_ = 0.0;
+        pw_last_err_ = 0.0;
+    }
+    const double gapv = std::fabs(std::log10(rel_du_ / rel_pr_));
+    if (gapv < best_gap_) {
+        best_gap_ = gapv;
+        best_pw_ = pw_;
+    }
+    HIP_OK(hipMemcpyAsync(x0_, xp_, (size_t)n_ * sizeof(X), hipMemcpyDeviceToDevice, st_));
+    HIP_OK(hipMemcpyAsync(x_, xp_, (size_t)n_ * sizeof(X), hipMemcpyDeviceToDevice, st_));
+    HIP_OK(hipMemcpyAsync(y0_, yp_, (size_t)m_ * sizeof(X), hipMemcpyDeviceToDevice, st_));
+    HIP_OK(hipMemcpyAsync(y_, yp_, (size_t)m_ * sizeof(X), hipMemcpyDeviceToDevice, st_));
+    inner_count_ = 0;
+    last_trial_fpe_ = INFINITY;
+}
+
+// reference src/utils.cu:291-322
+template <class V, class X> bool Solver<V, X>::should_restart() {
+    const restart_parameters_t &rp = P_.restart_params;
+    const int ef = P_.termination_evaluation_frequency;
+    bool r = false;
+    if (total_count_ == ef) r = true;
+    else if (total_count_ > ef) {
+        if (fpe_ <= rp.sufficient_reduction_for_restart * init_fpe_) r = true;
+        if (fpe_ <= rp.necessary_reduction_for_restart * init_fpe_ && fpe_ > last_trial_fpe_) r = true;
+        if (inner_count_ >= rp.artificial_restart_threshold * total_count_) r = true;
+    }
+    last_trial_fpe_ = fpe_;
+    return r;
+}
+
+// reference src/pdhg_core_op.cu:1037-1294
+template <class V, class X> void Solver<V, X>::residual() {
+    const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
+    const double tau = step_ / pw_;
+    spmv_launch<V, X, Acc>(A_, pA_, xp_, EpiStore<X>{ax_}, st_);
+    spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
+    q_product(xp_, qx_, nullptr);
+    GridRed r0 = red();
+    hipLaunchKernelGGL(residual_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_, yp_,
+                       con_rescale_, cl_fin_, cu_fin_, m_, r0);
+    GridRed r1 = r0;
+    r1.out = ds_->red + 3;
+    hipLaunchKernelGGL(residual_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, aty_, qx_, xp_, lb_, ub_,
+                       dslack_, var_rescale_, tau, is_lp_ ? 0 : 1, n_, r1);
+    HIP_OK(hipGetLastError());
+    read_scalars();
+    const double *R = hs_->red;
+    // rows: [sum pres^2, row terms, max|pres|]; cols: [sum dres^2, c'x, x'Qx, slack'x, (Qx)^2, (A'y)^2, max dres, max Qx, max A'y]
+    abs_pr_ = (linf ? R[2] : std::sqrt(R[0])) / cbr_;
+    const double lin_pr = abs_pr_;
+    abs_du_ = (linf ? R[3 + 6] : std::sqrt(R[3 + 0])) / ovr_;
+    const double half_xqx = qtype_ == PDHCG_NON_Q ? 0.0 : 0.5 * R[3 + 2];
+    pobj_ = (R[3 + 1] + half_xqx) / (cbr_ * ovr_) + obj_const_;
+    dobj_ = (R[3 + 3] + R[1] - half_xqx) / (cbr_ * ovr_) + obj_const_;
+    rel_pr_ = lin_pr / (1.0 + bound_norm_);
+    double den;
+    if (is_lp_) den = 1.0 + obj_norm_;
+    else {
+        const double qn = linf ? R[3 + 7] : std::sqrt(R[3 + 4]);
+        const double an = linf ? R[3 + 8] : std::sqrt(R[3 + 5]);
+        den = 1.0 + std::fmax(obj_norm_, std::fmax(qn / ovr_, an / ovr_));
+    }
+    rel_du_ = abs_du_ / den;
+    gap_ = std::fabs(pobj_ - dobj_);
+    rel_gap_ = gap_ / (1.0 + std::fabs(pobj_) + std::fabs(dobj_));
+}
+
+// reference src/pdhg_core_op.cu:1296-1429
+template <class V, class X> void Solver<V, X>::infeasibility() {
+    hipLaunchKernelGGL(ray_project_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, dx_, lb_, ub_, dy_,
+                       cl_, cu_, n_, m_, red());
+    read_scalars();
+    const double prn = hs_->red[0], drn = hs_->red[1];
+    if (prn > 0.0) hipLaunchKernelGGL(scale_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, dx_, 1.0 / prn, n_);
+    spmv_launch<V, X, Acc>(A_, pA_, dx_, EpiStore<X>{ax_}, st_);
+    spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
+    GridRed r0 = red();
+    hipLaunchKernelGGL(ray_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, dy_, ax_, g_, cl_, cu_, cl_fin_,
+                       cu_fin_, con_rescale_, m_, r0);
+    GridRed r1 = r0;
+    r1.out = ds_->red + 2;
+    hipLaunchKernelGGL(ray_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, dx_, aty_, lb_, ub_, lb_fin_,
+                       ub_fin_, var_rescale_, n_, r1);
+    double qn = 0.0;
+    const bool has_q = !is_lp_ && qtype_ != PDHCG_NON_Q;
+    if (has_q) {
+        q_product(dx_, qx_, nullptr);
+        GridRed r2 = r0;
+        r2.out = ds_->red + 5;
+        hipLaunchKernelGGL(inf_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, qx_, n_, r2);
+    }
+    HIP_OK(hipGetLastError());
+    read_scalars();
+    const double *R = hs_->red;
+    pray_lin_obj_ = R[2] / (cbr_ * ovr_);
+    dray_obj_ = (R[0] + R[3]) / (cbr_ * ovr_);
+    max_pray_ = R[1];
+    if (has_q) {
+        qn = R[5];
+        max_pray_ = std::fmax(max_pray_, qn / ovr_);
+    }
+    const double dsn = R[4];
+    max_dray_ = dsn;
+    const double sc = std::fmax(drn, dsn);
+    if (sc > 0.0) {
+        max_dray_ /= sc;
+        dray_obj_ /= sc;
+    } else {
+        max_dray_ = 0.0;
+        dray_obj_ = 0.0;
+    }
+}
+
+// reference src/utils.cu:219-289
+template <class V, class X> void Solver<V, X>::check_termination() {
+    const termination_criteria_t &c = P_.termination_criteria;
+    if (g_pdhcg_cancel) {
+        reason_ = TERMINATION_REASON_USER_INTERRUPT;
+        return;
+    }
+    if (rel_du_ < c.eps_feasible_relative && rel_pr_ < c.eps_feasible_relative && rel_gap_ < c.eps_optimal_relative) {
+        reason_ = TERMINATION_REASON_OPTIMAL;
+        return;
+    }
+    if (dray_obj_ > 0.0 && max_dray_ / dray_obj_ <= c.eps_infeasible) {
+        reason_ = TERMINATION_REASON_PRIMAL_INFEASIBLE;
+        return;
+    }
+    if (pray_lin_obj_ < 0.0 && max_pray_ / (-pray_lin_obj_) <= c.eps_infeasible) {
+        reason_ = TERMINATION_REASON_DUAL_INFEASIBLE;
+        return;
+    }
+    if (total_count_ >= c.iteration_limit) {
+        reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        return;
+    }
+    if (cum_time_ >= c.time_sec_limit) reason_ = TERMINATION_REASON_TIME_LIMIT;
+}
+
+// Main loop body, reference src/solver.cu:121-169.
+template <class V, class X> int Solver<V, X>::run(int iterations) {
+    const int ef = P_.termination_evaluation_frequency;
+    for (int done = 0; done < iterations; ++done) {
+        if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        if (!(total_count_ < P_.termination_criteria.iteration_limit)) {
+            reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+            residual();
+            break;
+        }
+        const int tc = total_count_;
+        if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
+            residual();
+            if (is_major_ && tc < 3 * ef) infeasibility();
+            cum_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start_).count();
+            check_termination();
+            if (P_.verbose >= 2 && tc % print_frequency(tc) == 0)
+                std::printf("%6d %7d %.1e | %8.1e  %8.1e | %.1e %.1e %.1e | %.1e %.1e %.1e \n", tc,
+                            host_inner_ + hs_->inner_total, cum_time_, pobj_, dobj_, abs_pr_, abs_du_, gap_, rel_pr_,
+                            rel_du_, rel_gap_);
+            if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+        }
+        if (is_major_ || tc == 0) {
+            do_restart_ = should_restart();
+            if (do_restart_) restart();
+        }
+        is_major_ = ((tc + 1) % ef) == 0;
+        const bool need_fpe = is_major_ || do_restart_;
+        const double w = (double)(inner_count_ + 1) / (inner_count_ + 2);
+        pdhg_step(!need_fpe);
+        if (need_fpe) {
+            fixed_point_error();
+            if (do_restart_) {
+                init_fpe_ = fpe_;
+                do_restart_ = false;
+            }
+            halpern(w);
+        }
+        inner_count_++;
+        total_count_++;
+    }
+    HIP_OK(hipGetLastError());
+    return (int)reason_;
+}
+
+// reference src/pdhg_core_op.cu:1431-1496
+template <class V, class X> pdhcg_result_t *Solver<V, X>::result(const qp_problem_t *orig) {
+    spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
+    q_product(xp_, qx_, nullptr);
+    double *xo = dalloc<double>(n_), *yo = dalloc<double>(m_), *rco = dalloc<double>(n_);
+    hipLaunchKernelGGL(result_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, yp_, c_,
+                       is_lp_ ? (const X *)nullptr : (const X *)qx_, aty_, lb_, ub_, var_rescale_, con_rescale_, ovr_,
+                       cbr_, n_, m_, xo, yo, rco);
+    HIP_OK(hipGetLastError());
+    pdhcg_result_t *r = ccalloc<pdhcg_result_t>(1);
+    r->primal_solution = cmalloc<double>(n_);
+    r->dual_solution = cmalloc<double>(m_);
+    r->reduced_cost = cmalloc<double>(n_);
+    HIP_OK(hipMemcpyAsync(r->primal_solution, xo, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipMemcpyAsync(r->dual_solution, yo, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipMemcpyAsync(r->reduced_cost, rco, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+    read_scalars();
+    (void)hipFree(xo);
+    (void)hipFree(yo);
+    (void)hipFree(rco);
+    r->num_variables = orig->num_variables;
+    r->num_constraints = orig->num_constraints;
+    r->num_nonzeros = orig->constraint_matrix_num_nonzeros;
+    r->total_count = total_count_;
+    r->total_inner_count = host_inner_ + hs_->inner_total;
+    r->rescaling_time_sec = rescale_time_;
+    r->cumulative_time_sec = cum_time_;
+    r->relative_primal_residual = rel_pr_;
+    r->relative_dual_residual = rel_du_;
+    r->absolute_primal_residual = abs_pr_;
+    r->absolute_dual_residual = abs_du_;
+    r->primal_objective_value = pobj_;
+    r->dual_objective_value = dobj_;
+    r->objective_gap = gap_;
+    r->relative_objective_gap = rel_gap_;
+    r->max_primal_ray_infeasibility = max_pray_;
+    r->max_dual_ray_infeasibility = max_dray_;
+    r->primal_ray_linear_objective = pray_lin_obj_;
+    r->dual_ray_objective = dray_obj_;
+    r->termination_reason = reason_;
+    return r;
+}
+
+template class Solver<double, double>;
+template class Solver<float, float>;
+
+}  // namespace pdhcg
+
+using namespace pdhcg;
+
+namespace {
+int g_default_precision = -1;  // -1: read PDHCG_PRECISION once
+
+int default_precision() {
+    if (g_default_precision < 0) {
+        const char *e = std::getenv("PDHCG_PRECISION");
+        g_default_precision = (e && (std::strcmp(e, "mixed") == 0 || std::strcmp(e, "fp32") == 0)) ? PDHCG_PRECISION_MIXED
+                                                                                                  : PDHCG_PRECISION_FP64;
+    }
+    return g_default_precision;
+}
+
+const char *reason_name(int r) {
+    static const char *names[] = {"UNSPECIFIED",  "OPTIMAL",         "PRIMAL_INFEASIBLE", "DUAL_INFEASIBLE",
+                                  "INFEASIBLE_OR_UNBOUNDED", "TIME_LIMIT", "ITERATION_LIMIT", "USER_INTERRUPT",
+                                  "FEAS_POLISH_SUCCESS"};
+    return (r >= 0 && r < 9) ? names[r] : "UNKNOWN";
+}
+
+void final_log(const pdhcg_result_t *r, const pdhg_parameters_t &p) {  // reference src/utils.cu:487-510
+    if (p.verbose < 1) return;
+    std::printf("Solution Summary\n");
+    std::printf("  Status             : %s\n", reason_name(r->termination_reason));
+    std::printf("  Solve time         : %.3g sec\n", r->cumulative_time_sec);
+    std::printf("  Iterations         : %d\n", r->total_count);
+    if (r->total_inner_count > 0) std::printf("  Inner Iterations   : %d\n", r->total_inner_count);
+    std::printf("  Primal objective   : %.10g\n", r->primal_objective_value);
+    std::printf("  Dual objective     : %.10g\n", r->dual_objective_value);
+    std::printf("  Objective gap      : %.3e\n", r->relative_objective_gap);
+    std::printf("  Primal infeas      : %.3e\n", r->relative_primal_residual);
+    std::printf("  Dual infeas        : %.3e\n", r->relative_dual_residual);
+}
+}  // namespace
+
+struct pdhcg_solver {
+    std::unique_ptr<SolverBase> impl;
+    const qp_problem_t *problem;
+    pdhg_parameters_t params;
+};
+
+extern "C" {
+
+void pdhcg_set_precision(int precision) {
+    g_default_precision = precision == PDHCG_PRECISION_MIXED ? PDHCG_PRECISION_MIXED : PDHCG_PRECISION_FP64;
+}
+
+int pdhcg_get_precision(void) { return default_precision(); }
+
+void pdhcg_request_cancel(int flag) { g_pdhcg_cancel = flag; }
+
+pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params, int precision) {
+    if (!prob) {
+        std::fprintf(stderr, "[interface] pdhcg_solver_create: invalid arguments.\n");
+        return nullptr;
+    }
+    if (prob->cones.num_cones > 0 || prob->affine_cones.num_cones > 0) {
+        std::fprintf(stderr, "[interface] cone blocks are not yet supported by this build's device solver.\n");
+        return nullptr;
+    }
+    if (prob->num_quadratic_constraints > 0) {
+        std::fprintf(stderr, "Error: QCQP -> SOCP transformation is not available; cannot solve.\n");
+        return nullptr;
+    }
+    try {
+        auto *s = new pdhcg_solver();
+        s->problem = prob;
+        if (params) s->params = *params;
+        else fill_default_parameters(&s->params);
+        ScaledProblem sp = scale_problem(s->params, *prob);
+        if (precision == PDHCG_PRECISION_MIXED) s->impl.reset(new Solver<float, float>(sp, s->params));
+        else s->impl.reset(new Solver<double, double>(sp, s->params));
+        return s;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] solver setup failed: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+int pdhcg_solver_run(pdhcg_solver_t *s, int iterations) {
+    if (!s) return -1;
+    try {
+        return s->impl->run(iterations);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] solver step failed: %s\n", e.what());
+        return -1;
+    }
+}
+
+int pdhcg_solver_counts(pdhcg_solver_t *s, int *total, int *inner) {
+    if (!s) return -1;
+    try {
+        s->impl->counts(total, inner);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s\n", e.what());
+        return -1;
+    }
+}
+
+int pdhcg_solver_synchronize(pdhcg_solver_t *s) {
+    if (!s) return -1;
+    try {
+        s->impl->synchronize();
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s\n", e.what());
+        return -1;
+    }
+}
+
+pdhcg_result_t *pdhcg_solver_result(pdhcg_solver_t *s) {
+    if (!s) return nullptr;
+    try {
+        return s->impl->result(s->problem);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] result extraction failed: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void pdhcg_solver_free(pdhcg_solver_t *s) {
+    if (!s) return;
+    try {
+        delete s;
+    } catch (...) {
+    }
+}
+
+pdhcg_result_t *solve_qp_problem(const qp_problem_t *prob, const pdhg_parameters_t *params) {
+    if (!prob) {
+        std::fprintf(stderr, "[interface] solve_qp_problem: invalid arguments.\n");
+        return nullptr;
+    }
+    if (validate_fixed_sections(prob) != 0) return nullptr;
+    pdhg_parameters_t local;
+    if (params) local = *params;
+    else fill_default_parameters(&local);
+    pdhcg_solver_t *s = pdhcg_solver_create(prob, &local, default_precision());
+    if (!s) {
+        std::fprintf(stderr, "[interface] optimize returned NULL.\n");
+        return nullptr;
+    }
+    int rc = 0;
+    while (rc == 0) rc = pdhcg_solver_run(s, 1 << 30);
+    pdhcg_result_t *r = rc < 0 ? nullptr : pdhcg_solver_result(s);
+    pdhcg_solver_free(s);
+    if (!r) {
+        std::fprintf(stderr, "[interface] optimize returned NULL.\n");
+        return nullptr;
+    }
+    final_log(r, local);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+"""Seeded synthetic problem families used by the benchmark and the tests.
+
+config0 : small random convex QP (sparse Q = B'B + diag), n=2000, m=1000.
+config1 : large sparse QP with a dense low-rank term, power-law (Zipf 1.5)
+          row lengths in A, diagonal Q, R (k x n) dense N(0, 1/n), diagonal D.
+Both follow the instance descriptions of the benchmark configurations, with
+the documented deviation that x0 ~ U[0, 5] for config1 (a start inside the
+[0, 5] box, so that the row bounds built around A x0 are feasible).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from .problem import QPData
+
+
+def _unique_columns_per_row(rng, lengths: np.ndarray, n: int) -> np.ndarray:
+    """Column indices, uniform without replacement within each row, rows in order."""
+    nnz = int(lengths.sum())
+    rows = np.repeat(np.arange(lengths.size, dtype=np.int64), lengths)
+    cols = rng.integers(0, n, nnz, dtype=np.int64)
+    for _ in range(64):
+        key = rows * n + cols
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        dup = np.zeros(nnz, dtype=bool)
+        dup[order[1:][ks[1:] == ks[:-1]]] = True
+        if not dup.any():
+            break
+        cols[dup] = rng.integers(0, n, int(dup.sum()), dtype=np.int64)
+    # sort columns inside each row
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    return cols[order].astype(np.int32)
+
+
+def random_sparse(rng, m: int, n: int, nnz: int) -> sp.csr_matrix:
+    """Exactly nnz positions uniform without replacement, N(0,1) values."""
+    flat = rng.choice(m * n, size=nnz, replace=False)
+    r, c = np.divmod(flat, n)
+    return sp.csr_matrix((rng.standard_normal(nnz), (r, c)), shape=(m, n))
+
+
+def config0(seed: int = 1000, n: int = 2000, m: int = 1000) -> QPData:
+    """Plain FP64 small random convex QP (tolerance 1e-4 in the benchmark)."""
+    rng = np.random.default_rng(seed)
+    A = random_sparse(rng, m, n, int(round(0.005 * m * n)))
+    kb = max(1, n // 10)
+    B = random_sparse(rng, kb, n, int(round(0.01 * kb * n)))
+    Q = (B.T @ B + sp.diags(rng.uniform(0.1, 1.0, n))).tocsr()
+    Q.sort_indices()
+    c = rng.standard_normal(n)
+    x0 = rng.uniform(-1.0, 1.0, n)
+    ax = A @ x0
+    return QPData(c=c, A=A, Q=Q, con_lb=ax - rng.uniform(0, 1, m), con_ub=ax + rng.uniform(0, 1, m),
+                  var_lb=np.full(n, -10.0), var_ub=np.full(n, 10.0))
+
+
+def zipf_lengths(rng, m: int, nnz: int, a: float = 1.5, max_len: int = 5000) -> np.ndarray:
+    """Row lengths ~ Zipf(a) truncated to [1, max_len], rescaled to sum exactly nnz."""
+    raw = rng.zipf(a, size=m)
+    while True:
+        bad = raw > max_len
+        if not bad.any():
+            break
+        raw[bad] = rng.zipf(a, size=int(bad.sum()))
+    lens = np.maximum(1, np.round(raw * (nnz / raw.sum()))).astype(np.int64)
+    lens = np.minimum(lens, max_len)
+    diff = nnz - int(lens.sum())
+    order = np.argsort(-lens, kind="stable")
+    i = 0
+    while diff != 0:
+        r = order[i % m]
+        step = 1 if diff > 0 else -1
+        if 1 <= lens[r] + step <= max_len:
+            lens[r] += step
+            diff -= step
+        i += 1
+    return lens
+
+
+def config1(scale: float = 1.0, seed: int = 1001, rank: int = 50) -> QPData:
+    """Large sparse QP + dense low-rank term; scale shrinks n, m and nnz together."""
+    rng = np.random.default_rng(seed)
+    n = max(10, int(round(1_000_000 * scale)))
+    m = max(5, int(round(500_000 * scale)))
+    nnz = max(m, int(round(10_000_000 * scale)))
+    lens = zipf_lengths(rng, m, nnz, 1.5, min(5000, n))
+    cols = _unique_columns_per_row(rng, lens, n)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(lens, out=rp[1:])
+    vals = rng.standard_normal(int(rp[-1]))
+    A = sp.csr_matrix((vals, cols, rp.astype(np.int32)), shape=(m, n))
+    qd = rng.uniform(0.0, 1.0, n)
+    Q = sp.diags(qd).tocsr()
+    R = rng.standard_normal((rank, n)) * np.sqrt(1.0 / n)
+    D = rng.uniform(0.5, 2.0, rank)
+    c = rng.standard_normal(n)
+    x0 = rng.uniform(0.0, 5.0, n)
+    ax = A @ x0
+    return QPData(c=c, A=A, Q=Q, R=R, D=D, con_lb=ax - rng.uniform(0, 1, m), con_ub=ax + rng.uniform(0, 1, m),
+                  var_lb=np.zeros(n), var_ub=np.full(n, 5.0), fmt={"R": "dense"})
+
+
+def random_problem(kind: str, seed: int = 0, n: int = 60, m: int = 30) -> QPData:
+    """Small random instances of each quadratic-objective type."""
+    rng = np.random.default_rng(seed)
+    A = random_sparse(rng, m, n, max(1, int(0.2 * m * n)))
+    x0 = rng.uniform(-1, 1, n)
+    ax = A @ x0
+    base = dict(c=rng.standard_normal(n), A=A, con_lb=ax - rng.uniform(0, 1, m), con_ub=ax + rng.uniform(0, 1, m),
+                var_lb=np.full(n, -10.0), var_ub=np.full(n, 10.0))
+    if kind == "lp":
+        return QPData(**base)
+    if kind == "diag":
+        return QPData(Q=sp.diags(rng.uniform(0.1, 2.0, n)).tocsr(), **base)
+    if kind == "sparse":
+        B = random_sparse(rng, max(2, n // 5), n, max(1, int(0.3 * (n // 5) * n)))
+        return QPData(Q=(B.T @ B + sp.diags(rng.uniform(0.1, 1.0, n))).tocsr(), **base)
+    if kind == "lowrank":
+        return QPData(R=rng.standard_normal((6, n)) * 0.5, fmt={"R": "dense"}, **base)
+    if kind == "lowrank_diag":
+        return QPData(Q=sp.diags(rng.uniform(0.0, 1.0, n)).tocsr(), R=rng.standard_normal((5, n)) / np.sqrt(n),
+                      D=rng.uniform(0.5, 2.0, 5), fmt={"R": "dense"}, **base)
+    if kind == "lowrank_dense_d":
+        M = rng.standard_normal((4, 4))
+        return QPData(R=rng.standard_normal((4, n)) * 0.5, D=M.T @ M + 0.2 * np.eye(4), fmt={"R": "dense"}, **base)
+    if kind == "lowrank_sparse_r":
+        # R with 5% density: stored as CSR (R and R') by the device solver
+        R = random_sparse(rng, 8, n, max(1, int(0.05 * 8 * n)))
+        B = random_sparse(rng, max(2, n // 20), n, max(1, int(0.05 * (n // 20) * n)))
+        return QPData(Q=(B.T @ B + sp.diags(rng.uniform(0.1, 1.0, n))).tocsr(), R=R, **base)
+    raise ValueError(kind)
