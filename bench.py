@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: restarted PDHCG iterations/s on one MI355X.
+
+Workload: the large sparse QP with a dense low-rank term (n = 1e6 variables,
+m = 5e5 rows, nnz(A) = 1e7 with Zipf(1.5) row lengths, diagonal Q, R 50 x n
+dense N(0, 1/n), diagonal D, box [0, 5]), generated from a fixed seed.  A
+"step" is one pass of the solver's main loop: one PDHG iteration with its
+projected-BB inner solve, plus the KKT evaluation / restart work the loop does
+at that iteration.  The timed region holds exactly --steps steps between two
+device synchronisations, inputs already resident in HBM.  After the timed
+steps the same solver continues to the 1e-6 relative KKT tolerance, which
+gives the time-to-tolerance figure.
+
+Also reported: the achieved HBM bandwidth of the dominant kernel (HIP events
+on the solver stream; algorithmic bytes per launch / mean launch time), and
+the FP64 CPU oracle's iteration rate on a bounded sample of the same problem.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+METRIC = "time-to-1e-6 KKT & PDHG iter/s: plain vs mixed/batched/warm; % HBM roofline"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(p, precision):
+    """Algorithmic (compulsory) bytes per launch of the instrumented kernels."""
+    import numpy as np
+    sv = sx = 8 if precision == "fp64" else 4
+    si = 4
+    n = p.c.size
+    m = p.A.shape[0]
+    k = p.R.shape[0]
+    nnz = p.A.nnz
+    return {
+        # R (k x n) + x
+        0: k * n * sv + n * sx,
+        # R + x + diag(Q) + write Qx
+        1: k * n * sv + 3 * n * sx,
+        # A' SpMV (n rows of A'): values+indices, row pointers, gather y once, write A'y
+        2: nnz * (sv + si) + (n + 1) * si + m * sx + n * sx,
+        # A SpMV with dual epilogue: values+indices, row ptrs, gather x_bar once,
+        # read y, g, l, u, y0 and write y (fused Halpern) -> 6 m-vectors
+        3: nnz * (sv + si) + (m + 1) * si + n * sx + 6 * m * sx,
+    }
+
+
+def run_gpu(p, precision, steps, warmup, eps, max_solve_s):
+    import ctypes as C
+    import pdhcg_amd
+    L = pdhcg_amd.lib()
+    L.pdhcg_solver_timing.restype = C.c_int
+    L.pdhcg_solver_timing.argtypes = [C.c_void_p, C.c_int]
+    L.pdhcg_solver_kernel_time.restype = C.c_int
+    L.pdhcg_solver_kernel_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    prec = pdhcg_amd.PRECISION_MIXED if precision == "mixed" else pdhcg_amd.PRECISION_FP64
+    t = time.time()
+    st = pdhcg_amd.Stepper(p, prec, eps=eps)
+    setup_s = time.time() - t
+    log(f"[bench] {precision}: solver setup {setup_s:.1f}s")
+    t_solve0 = time.perf_counter()
+    rc = st.run(warmup)
+    st.synchronize()
+    L.pdhcg_solver_timing(st.s, 1)
+    t0 = time.perf_counter()
+    rc = st.run(steps)
+    st.synchronize()
+    t1 = time.perf_counter()
+    kt = {}
+    for kid in range(4):
+        ms, cnt = C.c_double(), C.c_int()
+        L.pdhcg_solver_kernel_time(st.s, kid, C.byref(ms), C.byref(cnt))
+        kt[kid] = (ms.value, cnt.value)
+    L.pdhcg_solver_timing(st.s, 0)
+    total_steps, inner_steps = st.counts()
+    log(f"[bench] {precision}: {steps} steps in {t1 - t0:.3f}s (rc={rc}, total {total_steps}, inner {inner_steps})")
+    # continue to tolerance
+    while rc == 0 and time.perf_counter() - t_solve0 < max_solve_s:
+        rc = st.run(200)
+    st.synchronize()
+    t_end = time.perf_counter()
+    res = st.result()
+    st.close()
+    return dict(setup_s=setup_s, step_s=t1 - t0, kernel_times=kt, total_iters=res.iterations,
+                inner_iters=res.inner_iterations, status=res.status, time_to_tol=t_end - t_solve0,
+                rel=(res.rel_primal, res.rel_dual, res.rel_gap), pobj=res.pobj)
+
+
+def run_cpu_baseline(p, eps, budget_s):
+    """FP64 CPU oracle (single thread) on a bounded sample: setup once, then
+    as many main-loop iterations as fit in budget_s."""
+    import ctypes as C
+    from tests.oracle_lib import oracle, oracle_build, oracle_params
+    L = oracle()
+    h = oracle_build(p)
+    prm = oracle_params(eps=eps)
+    t = time.time()
+    s = L.orc_solver_create(h.ptr, C.byref(prm))
+    setup = time.time() - t
+    iters = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        L.orc_solver_run(s, 1)
+        iters += 1
+    dt = time.perf_counter() - t0
+    L.orc_solver_free(s)
+    h.close()
+    return dict(iters=iters, seconds=dt, setup_s=setup)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--compare", action="store_true", help="also run the other precision and report both")
+    ap.add_argument("--scale", type=float, default=1.0, help="problem size factor (1.0 = full workload)")
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--max-solve-seconds", type=float, default=60.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from pdhcg_amd import workloads
+    t = time.time()
+    p = workloads.config1(scale=args.scale)
+    log(f"[bench] workload generated in {time.time() - t:.1f}s: n={p.c.size} m={p.A.shape[0]} nnz={p.A.nnz} "
+        f"rank={p.R.shape[0]}")
+
+    runs = {args.precision: run_gpu(p, args.precision, args.steps, args.warmup, args.eps, args.max_solve_seconds)}
+    if args.compare:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        runs[other] = run_gpu(p, other, args.steps, args.warmup, args.eps, args.max_solve_seconds)
+    main_run = runs[args.precision]
+    ms_per_step = 1e3 * main_run["step_s"] / args.steps
+    value = args.steps / main_run["step_s"]
+
+    kb = kernel_bytes(p, args.precision)
+    kt = main_run["kernel_times"]
+    # dominant kernel = largest total recorded time among the instrumented ones
+    dom = max(kt, key=lambda k: kt[k][0])
+    names = {0: "rx_dense_kernel (R x, D)", 1: "rtu_dense_kernel (R'u + diag Q)",
+             2: "spmv A'y (PDHG primal step)", 3: "spmv A x_bar + dual prox/Halpern"}
+    ms, cnt = kt[dom]
+    avg_s = (ms / max(cnt, 1)) * 1e-3
+    achieved = kb[dom] / avg_s / 1e9 if avg_s > 0 else 0.0
+    kernels = {names[k]: {"launches": kt[k][1], "avg_us": 1e3 * kt[k][0] / max(kt[k][1], 1),
+                          "bytes_per_launch": kb[k],
+                          "GBps": (kb[k] / (kt[k][0] / max(kt[k][1], 1) * 1e-3) / 1e9) if kt[k][0] > 0 else None}
+               for k in kt}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        c = run_cpu_baseline(p, args.eps, args.cpu_seconds)
+        cpu = {"value": c["iters"] / c["seconds"], "unit": "iter/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle, {c['iters']} main-loop iterations of the same workload in {c['seconds']:.1f}s "
+                         f"(after {c['setup_s']:.1f}s setup), single thread"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "sparse QP + dense low-rank term: n=%d m=%d nnz(A)=%d Zipf(1.5) rows, diag Q, "
+                               "R %dx%d dense, diag D, box [0,5]" % (p.c.size, p.A.shape[0], p.A.nnz,
+                                                                        p.R.shape[0], p.c.size),
+                   "precision": args.precision, "scale": args.scale},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "time_to_tol_s": main_run["time_to_tol"],
+        "tol": args.eps,
+        "solve_status": main_run["status"],
+        "solve_iterations": main_run["total_iters"],
+        "solve_inner_iterations": main_run["inner_iters"],
+        "solve_rel_residuals": main_run["rel"],
+        "kernels": kernels,
+    }
+    if args.compare:
+        other = [k for k in runs if k != args.precision][0]
+        o = runs[other]
+        out["other_precision"] = {"precision": other, "iter_per_s": args.steps / o["step_s"],
+                                  "time_to_tol_s": o["time_to_tol"], "status": o["status"],
+                                  "iterations": o["total_iters"], "rel_residuals": o["rel"], "pobj": o["pobj"]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -45,10 +45,17 @@ template <int NS> __device__ __forceinline__ double combine(int i, double a, dou
 
 // Reduction request: the output receives NS sums followed by NM maxima (FP64).
 struct GridRed {
-    double *partials;   // [kMaxRedBlocks * kMaxRedValues]
+    double *partials;   // [kMaxRedValues][kMaxRedBlocks]
     unsigned *counter;  // arrival counter, zero between launches
     double *out;
 };
+
+// Load of a value written by another workgroup of the same launch: coherent
+// at device scope (bypasses a possibly stale per-CU cache line) but, unlike a
+// volatile access, free to be pipelined with other loads.
+__device__ __forceinline__ double coherent_load(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Block reduction of a value pack into thread 0 (fixed combination order).
 template <int NS, int NM> __device__ __forceinline__ void block_reduce(double (&v)[NS + NM], double *lds) {
@@ -85,7 +92,7 @@ template <int NS, int NM> __device__ bool grid_reduce(const GridRed &g, double (
     block_reduce<NS, NM>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) g.partials[(size_t)blockIdx.x * kMaxRedValues + i] = v[i];
+        for (int i = 0; i < NV; ++i) g.partials[(size_t)i * kMaxRedBlocks + blockIdx.x] = v[i];
         __threadfence();
         unsigned prev = atomicAdd(g.counter, 1u);
         last = (prev == gridDim.x - 1) ? 1 : 0;
@@ -93,13 +100,13 @@ template <int NS, int NM> __device__ bool grid_reduce(const GridRed &g, double (
     __syncthreads();
     if (!last) return false;
     __threadfence();
-    const volatile double *P = g.partials;
     double t[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) t[i] = i < NS ? 0.0 : -INFINITY;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
 #pragma unroll
-        for (int i = 0; i < NV; ++i) t[i] = combine<NS>(i, t[i], P[(size_t)b * kMaxRedValues + i]);
+        for (int i = 0; i < NV; ++i)
+            t[i] = combine<NS>(i, t[i], coherent_load(g.partials + (size_t)i * kMaxRedBlocks + b));
     block_reduce<NS, NM>(t, lds);
     if (threadIdx.x == 0) {
 #pragma unroll

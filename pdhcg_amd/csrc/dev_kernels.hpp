@@ -282,47 +282,43 @@ __global__ void __launch_bounds__(kBlock) bb_final_kernel(X *x, const X *xp, X *
 // Low-rank product R'DRx with R dense row-major (k x n)
 // ---------------------------------------------------------------------------
 
-constexpr int kRxColsPerThread = 4;
+constexpr int kRxChunk = 2048;  // columns of R per workgroup (x chunk staged in LDS)
 constexpr int kRxMaxRank = 256;
 
-// Per-block partial sums of R x over a column chunk; the last block folds the
-// partials in block order, applies D and writes u = D R x (FP64).
+// Per-workgroup partial sums of R x over a chunk of kRxChunk columns: the x
+// chunk is staged once in LDS, then each wave streams whole contiguous row
+// segments R[j, chunk] (coalesced, four loads in flight per lane) and reduces
+// them with one butterfly per (row, chunk).  The last workgroup folds the
+// partials in workgroup order, applies D and writes u = D R x (FP64).
 // D: dkind 1 diagonal, 2 dense (symmetric part taken from the upper triangle
 // of the row-major array, as the reference's lower-fill Dsymv on the
 // column-major view does, reference pdhg_core_op.cu:272-303).
-template <class V, class X>
+template <class V, class X, bool Gated>
 __global__ void __launch_bounds__(kBlock) rx_dense_kernel(const V *R, const X *x, int n, int k, double *partials,
                                                            unsigned *counter, double *u, const double *ddiag,
                                                            const double *ddense, int dkind, const int *gate) {
-    if (gate && *gate == 0) return;
-    __shared__ double lds[kRxMaxRank * (kBlock / kWave)];
+    if (Gated && *gate == 0) return;
+    __shared__ double xs[kRxChunk];
     __shared__ double rx[kRxMaxRank];
     __shared__ int last;
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const long long chunk = (long long)kBlock * kRxColsPerThread;
-    const long long base = blockIdx.x * chunk + threadIdx.x;
-    double xv[kRxColsPerThread];
-#pragma unroll
-    for (int p = 0; p < kRxColsPerThread; ++p) {
-        const long long col = base + (long long)p * kBlock;
-        xv[p] = col < n ? (double)x[col] : 0.0;
-    }
-    for (int j = 0; j < k; ++j) {
-        const V *row = R + (size_t)j * n;
-        double s = 0.0;
-#pragma unroll
-        for (int p = 0; p < kRxColsPerThread; ++p) {
-            const long long col = base + (long long)p * kBlock;
-            if (col < n) s += (double)row[col] * xv[p];
-        }
-        s = wave_sum(s);
-        if (lane == 0) lds[j * (kBlock / kWave) + wave] = s;
-    }
+    const long long c0 = (long long)blockIdx.x * kRxChunk;
+    const int len = (int)((n - c0) < kRxChunk ? (n - c0) : kRxChunk);
+    for (int t = threadIdx.x; t < len; t += kBlock) xs[t] = (double)x[c0 + t];
     __syncthreads();
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        double a = lds[j * (kBlock / kWave)];
-        for (int w = 1; w < kBlock / kWave; ++w) a += lds[j * (kBlock / kWave) + w];
-        partials[(size_t)blockIdx.x * k + j] = a;
+    for (int j = wave; j < k; j += kBlock / kWave) {
+        const V *row = R + (size_t)j * n + c0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int t = lane;
+        for (; t + 3 * kWave < len; t += 4 * kWave) {
+            s0 += (double)row[t] * xs[t];
+            s1 += (double)row[t + kWave] * xs[t + kWave];
+            s2 += (double)row[t + 2 * kWave] * xs[t + 2 * kWave];
+            s3 += (double)row[t + 3 * kWave] * xs[t + 3 * kWave];
+        }
+        for (; t < len; t += kWave) s0 += (double)row[t] * xs[t];
+        double s = wave_sum((s0 + s1) + (s2 + s3));
+        if (lane == 0) partials[(size_t)j * gridDim.x + blockIdx.x] = s;
     }
     __threadfence();
     __syncthreads();
@@ -330,11 +326,13 @@ __global__ void __launch_bounds__(kBlock) rx_dense_kernel(const V *R, const X *x
     __syncthreads();
     if (!last) return;
     __threadfence();
-    const volatile double *P = partials;
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    // fold: one wave per row j, lanes stride over the workgroups (coalesced)
+    for (int j = wave; j < k; j += kBlock / kWave) {
+        const double *P = partials + (size_t)j * gridDim.x;
         double a = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) a += P[(size_t)b * k + j];
-        rx[j] = a;
+        for (int b = lane; b < (int)gridDim.x; b += kWave) a += coherent_load(P + b);
+        a = wave_sum(a);
+        if (lane == 0) rx[j] = a;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += blockDim.x) {
@@ -353,11 +351,13 @@ __global__ void __launch_bounds__(kBlock) rx_dense_kernel(const V *R, const X *x
 }
 
 // qx = R'u (+ qx, or + qdiag .* x)  with R dense row-major, one thread per column.
-template <class V, class X>
+// Gated launches (inside the BB loop) are a separate instantiation so that
+// profiles separate them from the always-running ones.
+template <class V, class X, bool Gated>
 __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const double *u, int n, int k, X *qx,
                                                             const X *qdiag, const X *x, int accumulate,
                                                             const int *gate) {
-    if (gate && *gate == 0) return;
+    if (Gated && *gate == 0) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double s = 0.0;
         for (int j = 0; j < k; ++j) s += (double)R[(size_t)j * n + i] * u[j];

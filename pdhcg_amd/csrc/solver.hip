@@ -18,6 +18,7 @@
 
 #include "../../include/pdhcg_amd.h"
 #include "dev_kernels.hpp"
+#include "dev_timer.hpp"
 #include "host_scale.h"
 
 namespace pdhcg {
@@ -93,6 +94,7 @@ class SolverBase {
     virtual void counts(int *total, int *inner) = 0;
     virtual void synchronize() = 0;
     virtual double setup_seconds() const = 0;
+    virtual KernelTimer &timer() = 0;
 };
 
 template <class V, class X> class Solver final : public SolverBase {
@@ -108,6 +110,7 @@ template <class V, class X> class Solver final : public SolverBase {
     }
     void synchronize() override { HIP_OK(hipStreamSynchronize(st_)); }
     double setup_seconds() const override { return setup_seconds_; }
+    KernelTimer &timer() override { return timer_; }
 
   private:
     using Acc = typename std::conditional<std::is_same<V, double>::value || std::is_same<X, double>::value, double,
@@ -189,6 +192,7 @@ template <class V, class X> class Solver final : public SolverBase {
     termination_reason_t reason_ = TERMINATION_REASON_UNSPECIFIED;
     double cum_time_ = 0;
     std::chrono::steady_clock::time_point t_start_;
+    KernelTimer timer_;
 };
 
 template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params) : P_(params) {
@@ -277,8 +281,7 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
             for (int j = 0; j < k_; ++j)
                 for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e) Rd[(size_t)j * n_ + sp.R.ci[e]] += sp.R.v[e];
             Rd_ = dupload<V>(Rd, Rd.size());
-            rx_blocks_ = (int)std::max<long long>(1, ((long long)n_ + kBlock * kRxColsPerThread - 1) /
-                                                         (kBlock * kRxColsPerThread));
+            rx_blocks_ = (int)std::max<long long>(1, ((long long)n_ + kRxChunk - 1) / kRxChunk);
             rx_partials_ = dzeros<double>((size_t)rx_blocks_ * k_);
             rx_counter_ = dzeros<unsigned>(1);
         } else {
@@ -503,11 +506,23 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
         const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
         if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, x, EpiStore<X>{qx}, st_, gate);
         if (r_dense_) {
-            hipLaunchKernelGGL((rx_dense_kernel<V, X>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_, k_,
-                               rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
             const X *qd = (plus && !q_sparse_) ? qsdiag_ : nullptr;
-            hipLaunchKernelGGL((rtu_dense_kernel<V, X>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_, qx, qd, x,
-                               (plus && q_sparse_) ? 1 : 0, gate);
+            const int acc = (plus && q_sparse_) ? 1 : 0;
+            if (gate) {
+                hipLaunchKernelGGL((rx_dense_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_,
+                                   k_, rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, true>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_, qx,
+                                   qd, x, acc, gate);
+            } else {
+                const int t0 = timer_.start(kTimedRx, st_);
+                hipLaunchKernelGGL((rx_dense_kernel<V, X, false>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_,
+                                   k_, rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+                timer_.stop(kTimedRx, t0, st_);
+                const int t1 = timer_.start(kTimedRtu, st_);
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, false>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_,
+                                   qx, qd, x, acc, gate);
+                timer_.stop(kTimedRtu, t1, st_);
+            }
         } else {
             spmv_launch<V, X, Acc>(Rc_, pR_, x, EpiStore<X>{rxv_}, st_, gate);
             hipLaunchKernelGGL(apply_middle_kernel<X>, dim3(1), dim3(kBlock), 0, st_, rxv_, u_, k_, ddiag_, ddense_,
@@ -578,14 +593,20 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
     if (qtype_ == PDHCG_NON_Q || qtype_ == PDHCG_DIAG_Q) {
         EpiPrimalStep<X> epi{x_, xp_, xr_, dslack_, c_, lb_, ub_, x0_, qtype_ == PDHCG_DIAG_Q ? qdiag_ : nullptr,
                              tau, w, rho, store, fuse_halpern ? 1 : 0};
+        const int t = timer_.start(kTimedPrimalStep, st_);
         spmv_launch<V, X, Acc>(At_, pAt_, y_, epi, st_);
+        timer_.stop(kTimedPrimalStep, t, st_);
     } else {
+        const int t = timer_.start(kTimedPrimalStep, st_);
         spmv_launch<V, X, Acc>(At_, pAt_, y_, EpiStore<X>{aty_}, st_);
+        timer_.stop(kTimedPrimalStep, t, st_);
         bb_prox(tau, fuse_halpern, w);
     }
     host_inner_ += 1;
     EpiDualStep<X> depi{y_, yp_, yr_, g_, cl_, cu_, y0_, sigma, w, rho, store, fuse_halpern ? 1 : 0};
+    const int t = timer_.start(kTimedDualStep, st_);
     spmv_launch<V, X, Acc>(A_, pA_, xr_, depi, st_);
+    timer_.stop(kTimedDualStep, t, st_);
 }
 
 template <class V, class X> void Solver<V, X>::halpern(double w) {
@@ -978,6 +999,31 @@ pdhcg_result_t *pdhcg_solver_result(pdhcg_solver_t *s) {
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[interface] result extraction failed: %s\n", e.what());
         return nullptr;
+    }
+}
+
+int pdhcg_solver_timing(pdhcg_solver_t *s, int enable) {
+    if (!s) return -1;
+    try {
+        s->impl->synchronize();
+        s->impl->timer().reset();
+        s->impl->timer().enabled = enable != 0;
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s\n", e.what());
+        return -1;
+    }
+}
+
+int pdhcg_solver_kernel_time(pdhcg_solver_t *s, int kernel, double *total_ms, int *launches) {
+    if (!s || kernel < 0 || kernel >= kNumTimed || !total_ms || !launches) return -1;
+    try {
+        s->impl->synchronize();
+        s->impl->timer().totals(kernel, total_ms, launches);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s\n", e.what());
+        return -1;
     }
 }
 

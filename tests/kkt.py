@@ -50,8 +50,12 @@ def kkt(p, x, y):
     qx = _Qx(p, x)
     aty = A.T @ y[:m] if m > 0 else np.zeros(n)
     grad = c + qx - aty
-    # optimal box multiplier lam in N_[lb,ub](x): lam = grad where the bound allows it
-    lam = np.where(grad > 0, np.where(np.isfinite(lb), grad, 0.0), np.where(np.isfinite(ub), grad, 0.0))
+    # box multiplier: the gradient on variables at (within act_tol of) the bound
+    # it pushes against, zero on interior variables
+    act_tol = 1e-5 * (1.0 + np.abs(np.where(np.isfinite(lb), lb, 0.0)) + np.abs(np.where(np.isfinite(ub), ub, 0.0)))
+    at_lb = np.isfinite(lb) & (x - lb <= act_tol)
+    at_ub = np.isfinite(ub) & (ub - x <= act_tol)
+    lam = np.where((grad > 0) & at_lb, grad, 0.0) + np.where((grad < 0) & at_ub, grad, 0.0)
     dres = grad - lam
     den = 1.0 + max(np.max(np.abs(c)), np.max(np.abs(qx), initial=0.0), np.max(np.abs(aty), initial=0.0))
     rel_dual = np.max(np.abs(dres), initial=0.0) / den
