@@ -78,13 +78,13 @@ void orc_solver_free(orc_solver_t *s);
 void orc_solver_scalings(const orc_solver_t *s, double *con_rescale, double *var_rescale, double *cbr, double *ovr,
                          double *step_size, double *primal_weight);
 
-/* ---- cone projections on the scaled space, one block at a time ---- */
-/* Euclidean projection of a standard second-order cone block (v, w, z) of
- * length v_dim + 2 (reference src/kernels/pdhcg_soc_cone_kernels.cu:305-327). */
-void orc_project_soc(double *x, int v_dim);
-/* Euclidean projection of a rotated block (v, s, t)
- * (reference src/kernels/pdhcg_rsoc_cone_kernels.cu:488-531). */
-void orc_project_rsoc(double *x, int v_dim);
+/* ---- cone projections on the scaled space, one block at a time ----
+ * x holds the block's slots, d the block's variable scaling (the projection
+ * is onto D K; d = 1 gives the Euclidean projection onto K), alpha the power
+ * cone exponent.  Returns 0, or -1 for an unknown type. */
+int orc_project_block(int type, double *x, int v_dim, const double *d, double alpha);
+/* Dual-cone residual (rc - P_{K*}(rc)) scaled by d, written to out. */
+int orc_dual_residual_block(int type, const double *rc, int v_dim, const double *d, double alpha, double *out);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@
  * src/pdhg_core_op.cu, src/kernels/pdhcg_kernels.cu and src/utils.cu:219-322.
  * Reductions are plain sequential sums.
  */
+#include "orc_cones.h"
 #include "orc_internal.h"
 
 #include <math.h>
@@ -47,7 +48,41 @@ struct orc_solver {
     termination_reason_t reason;
     double cum_time;
     struct timespec t_start;
+    /* variable cones (reference internal/internal_types.h:106-127) */
+    const cone_blocks_t *cb;
+    int has_cones, has_power, has_psd;
+    double *proj_warm, *res_warm; /* per-block bisection warm starts */
+    double *eff;                  /* c + Qx, the effective objective gradient (QP with cones) */
+    double *snap;                 /* BB primal snapshot */
+    double *pgp;                  /* projected-gradient point */
 };
+
+/* Applies every variable cone block's projection to v (reference
+ * src/cone_dispatch.cu:38-57). */
+static void project_cones(orc_solver_t *s, double *v, double *warm) {
+    const cone_blocks_t *b = s->cb;
+    const double *d = s->w->var_rescale;
+    for (int k = 0; k < b->num_cones; ++k) {
+        int st = b->start_idx[k], vd = b->v_dim[k];
+        switch (b->type[k]) {
+        case CONE_STANDARD_SOC: orc_soc_project(v + st, vd, d + st, warm + k); break;
+        case CONE_ROTATED_SOC: orc_rsoc_project(v + st, vd, d + st, warm + k); break;
+        case CONE_EXPONENTIAL: orc_exp_project(v + st, d + st); break;
+        case CONE_POWER: orc_power_project(v + st, d + st, b->power_alpha[k]); break;
+        case CONE_PSD: orc_psd_project(v + st, vd); break;
+        default: break;
+        }
+    }
+}
+
+/* x_bar = 2 x_p - x on every cone slot (reference src/cone_dispatch.cu:118-135). */
+static void cone_reflection(orc_solver_t *s) {
+    const cone_blocks_t *b = s->cb;
+    for (int k = 0; k < b->num_cones; ++k) {
+        int st = b->start_idx[k], L = orc_cone_block_length(b, k);
+        for (int i = st; i < st + L; ++i) s->xr[i] = 2.0 * s->xp[i] - s->x[i];
+    }
+}
 
 static void *xm(size_t n) { void *p = malloc(n ? n : 1); if (!p) abort(); return p; }
 static double *zeros(int n) { double *p = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); if (!p) abort(); return p; }
@@ -147,6 +182,8 @@ static void obj_product(orc_solver_t *s, const double *x) {
         }
         break;
     }
+    if (s->eff)  /* reference src/pdhg_core_op.cu:405-412 */
+        for (int i = 0; i < n; ++i) s->eff[i] = w->c[i] + s->qx[i];
 }
 
 /* ---- spectral estimates: reference src/pdhg_core_op.cu:1498-1788 ---- */
@@ -253,15 +290,16 @@ static void init_inner(orc_solver_t *s) {
     if (tmin <= 0.0) tmin = 1e-9;
     quad_obj_type_t qt = s->w->qtype;
     s->inner_limit = lim;
-    if (qt == PDHCG_NON_Q || qt == PDHCG_DIAG_Q) return;
+    if (qt == PDHCG_NON_Q || (qt == PDHCG_DIAG_Q && !s->has_psd)) return;
     int n = s->n;
     s->bb = 1;
     s->pbuf = zeros(n);
     s->grad = zeros(n);
     s->dirn = zeros(n);
+    if (s->has_cones) s->snap = zeros(n);
     s->inner_tol = t0;
     s->inner_min_tol = tmin;
-    s->precond = pp->diag_jacobi_precond;
+    s->precond = pp->diag_jacobi_precond && !s->has_psd;
     if (s->precond) {
         s->diag_h = zeros(n);
         s->mdiag = zeros(n);
@@ -298,8 +336,10 @@ static void init_inner(orc_solver_t *s) {
 
 orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params) {
     if (!prob) return NULL;
-    if (prob->cones.num_cones > 0 || prob->affine_cones.num_cones > 0 || prob->num_quadratic_constraints > 0) {
-        fprintf(stderr, "[oracle] cone and quadratic-constraint problems are not restated in this oracle.\n");
+    int any_fixed = 0;
+    for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i) any_fixed |= prob->cones.is_fixed[i] != 0;
+    if (prob->affine_cones.num_cones > 0 || prob->num_quadratic_constraints > 0 || any_fixed) {
+        fprintf(stderr, "[oracle] affine cones, fixed cone sections and quadratic constraints are not restated.\n");
         return NULL;
     }
     orc_solver_t *s = (orc_solver_t *)calloc(1, sizeof(orc_solver_t));
@@ -364,6 +404,36 @@ orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_
     s->bound_norm = linf ? acc : sqrt(acc);
     s->best_gap = INFINITY;
     s->last_trial_fpe = INFINITY;
+    /* variable cones: layout, initial auxiliary values (reference
+       src/solver_state.cu:578-686) and the initial cone projection (:976-980) */
+    s->cb = &work->cones;
+    s->has_cones = work->cones.num_cones > 0;
+    for (int k = 0; k < work->cones.num_cones; ++k) {
+        s->has_power |= work->cones.type[k] == CONE_POWER;
+        s->has_psd |= work->cones.type[k] == CONE_PSD;
+    }
+    if (s->has_cones && w->qtype == PDHCG_DIAG_Q && !s->has_psd) {
+        fprintf(stderr, "[oracle] the diagonal-Q cone proximal step is not restated.\n");
+        orc_solver_free(s);
+        return NULL;
+    }
+    s->proj_warm = zeros(work->cones.num_cones);
+    s->res_warm = zeros(work->cones.num_cones);
+    s->pgp = zeros(n);
+    if (s->has_cones && w->qtype != PDHCG_NON_Q) s->eff = zeros(n);
+    for (int k = 0; k < work->cones.num_cones; ++k) {
+        int aux = work->cones.start_idx[k] + work->cones.v_dim[k];
+        double *arrs[4] = {s->x0, s->x, s->xp, s->xr};
+        if (work->cones.type[k] == CONE_STANDARD_SOC) {
+            double wv = -0.7071067811865475 * w->cbr * w->var_rescale[aux];
+            double zv = 0.7071067811865475 * w->cbr * w->var_rescale[aux + 1];
+            for (int a = 0; a < 4; ++a) { arrs[a][aux] = wv; arrs[a][aux + 1] = zv; }
+        } else if (work->cones.type[k] == CONE_ROTATED_SOC) {
+            double tv = w->cbr * w->var_rescale[aux + 1];
+            for (int a = 0; a < 4; ++a) arrs[a][aux + 1] = tv;
+        }
+    }
+    if (s->has_cones) project_cones(s, s->x0, s->proj_warm);
     /* initial projection onto the box: reference src/solver_state.cu:981-995 */
     for (int j = 0; j < n; ++j) s->x0[j] = fmax(w->lb[j], fmin(s->x0[j], w->ub[j]));
     memcpy(s->x, s->x0, (size_t)n * sizeof(double));
@@ -422,6 +492,10 @@ static void bb_prox(orc_solver_t *s, double tau) {
         s->grad[i] = g;
         s->dirn[i] = nx - s->x[i];
     }
+    if (s->has_cones) { /* reference :561-566 */
+        project_cones(s, s->xp, s->proj_warm);
+        for (int i = 0; i < n; ++i) s->dirn[i] = s->xp[i] - s->x[i];
+    }
     double stms = 0.0, nrm = 0.0;
     while (it < s->inner_limit) {
         if (s->precond) {
@@ -440,6 +514,7 @@ static void bb_prox(orc_solver_t *s, double tau) {
             s->grad[i] = cur;
         }
         double tmp = dot(n, s->dirn, s->pbuf);
+        if (s->snap) memcpy(s->snap, s->xp, (size_t)n * sizeof(double));
         double alpha = s->precond ? stms / tmp : (nrm * nrm) / tmp;
         for (int i = 0; i < n; ++i) {
             double cur = s->xp[i];
@@ -447,6 +522,10 @@ static void bb_prox(orc_solver_t *s, double tau) {
             nx = clampd(nx, w->lb[i], w->ub[i]);
             s->dirn[i] = nx - cur;
             s->xp[i] = nx;
+        }
+        if (s->snap) { /* reference :652-657 */
+            project_cones(s, s->xp, s->proj_warm);
+            for (int i = 0; i < n; ++i) s->dirn[i] = s->xp[i] - s->snap[i];
         }
         it++;
     }
@@ -468,19 +547,29 @@ static void pdhg_step(orc_solver_t *s) {
     int tc = s->total_count;
     int store = s->is_major || ((tc + 2) % print_freq(tc + 2)) == 0;
     switch (w->qtype) {
-    case PDHCG_NON_Q:
+    case PDHCG_NON_Q: {
+        int store_x = store || s->has_cones; /* cones force the stored form (:428) */
         for (int i = 0; i < n; ++i) {
             double xi = s->x[i];
             double t = xi - tau * (w->c[i] - s->aty[i]);
             double pj = clampd(t, w->lb[i], w->ub[i]);
             s->xr[i] = 2.0 * pj - xi;
-            if (store) {
+            if (store_x) {
                 s->xp[i] = pj;
                 s->dslack[i] = (pj - t) / tau;
             }
         }
+        if (s->has_cones) { /* reference :730-735 */
+            project_cones(s, s->xp, s->proj_warm);
+            cone_reflection(s);
+        }
         break;
+    }
     case PDHCG_DIAG_Q:
+        if (s->has_psd) { /* reference :740-741 */
+            bb_prox(s, tau);
+            break;
+        }
         for (int i = 0; i < n; ++i) {
             double xi = s->x[i];
             double t = (xi - tau * (w->c[i] - s->aty[i])) / (1.0 + tau * w->qdiag[i]);
@@ -612,16 +701,70 @@ static void residual(orc_solver_t *s) {
             s->dslack[j] = ds;
         }
     }
+    const double *eff = s->eff ? s->eff : w->c;
+    if (s->has_cones) {
+        /* dual-cone residual of c_eff - A'y on the cone slots (reference
+           src/cone_dispatch.cu:88-115) */
+        const cone_blocks_t *b = s->cb;
+        double *rc = s->pgp;
+        for (int k = 0; k < b->num_cones; ++k) {
+            int st = b->start_idx[k], vd = b->v_dim[k], L = orc_cone_block_length(b, k);
+            for (int i = st; i < st + L; ++i) rc[i] = eff[i] - s->aty[i];
+            const double *d = w->var_rescale + st;
+            switch (b->type[k]) {
+            case CONE_STANDARD_SOC: orc_soc_dual_residual(rc + st, vd, d, s->res_warm + k, s->dres + st); break;
+            case CONE_ROTATED_SOC: orc_rsoc_dual_residual(rc + st, vd, d, s->res_warm + k, s->dres + st); break;
+            case CONE_EXPONENTIAL: orc_exp_dual_residual(rc + st, d, s->dres + st); break;
+            case CONE_POWER: orc_power_dual_residual(rc + st, d, b->power_alpha[k], s->dres + st); break;
+            case CONE_PSD: orc_psd_dual_residual(rc + st, vd, d, s->dres + st); break;
+            default: break;
+            }
+        }
+        /* projected-gradient mapping (reference src/pdhg_core_op.cu:67-90,
+           kernels :75-109) */
+        double st = s->step_size / s->primal_weight;
+        if (!(st > 0.0) || !isfinite(st)) st = 1.0;
+        for (int j = 0; j < n; ++j) s->pgp[j] = fmax(w->lb[j], fmin(s->xp[j] - st * (eff[j] - s->aty[j]), w->ub[j]));
+        project_cones(s, s->pgp, s->res_warm);
+        for (int j = 0; j < n; ++j) {
+            double r = (s->xp[j] - s->pgp[j]) / st * w->var_rescale[j];
+            if (!isfinite(r)) r = copysign(INFINITY, r);
+            if (fabs(r) > fabs(s->dres[j])) s->dres[j] = r;
+        }
+    }
     s->abs_pr = (linf ? inf_norm(m, s->pres) : nrm2(m, s->pres)) / w->cbr;
     double lin_pr = s->abs_pr;
+    double pv_abs = 0.0, pv_rel = 0.0;
+    if (s->has_power) { /* reference src/pdhg_core_op.cu:214-270 */
+        const cone_blocks_t *b = s->cb;
+        for (int k = 0; k < b->num_cones; ++k) {
+            if (b->type[k] != CONE_POWER) continue;
+            int st = b->start_idx[k];
+            double a, r;
+            orc_power_violation(s->xp + st, w->var_rescale + st, b->power_alpha[k], w->cbr, &a, &r);
+            if (linf) { pv_abs = fmax(pv_abs, a); pv_rel = fmax(pv_rel, r); }
+            else { pv_abs += a * a; pv_rel += r * r; }
+        }
+        if (!linf) { pv_abs = sqrt(pv_abs); pv_rel = sqrt(pv_rel); }
+        pv_abs /= w->cbr;
+        s->abs_pr = linf ? fmax(s->abs_pr, pv_abs) : hypot(s->abs_pr, pv_abs);
+    }
     s->abs_du = (linf ? inf_norm(n, s->dres) : nrm2(n, s->dres)) / w->ovr;
     double half_xqx = w->qtype == PDHCG_NON_Q ? 0.0 : 0.5 * dot(n, s->xp, s->qx);
     s->pobj = (dot(n, w->c, s->xp) + half_xqx) / (w->cbr * w->ovr) + w->obj_const;
+    if (s->has_cones) { /* cone dual slack (reference src/cone_dispatch.cu:137-154) */
+        const cone_blocks_t *b = s->cb;
+        for (int k = 0; k < b->num_cones; ++k) {
+            int st = b->start_idx[k], L = orc_cone_block_length(b, k);
+            for (int i = st; i < st + L; ++i) s->dslack[i] = eff[i] - s->aty[i];
+        }
+    }
     double base = dot(n, s->dslack, s->xp);
     double sps = 0.0;
     for (int i = 0; i < m; ++i) sps += s->pslack[i];
     s->dobj = (base + sps - half_xqx) / (w->cbr * w->ovr) + w->obj_const;
     s->rel_pr = lin_pr / (1.0 + s->bound_norm);
+    if (s->has_power) s->rel_pr = linf ? fmax(s->rel_pr, pv_rel) : hypot(s->rel_pr, pv_rel);
     double den;
     if (s->ptype == P_LP) den = 1.0 + s->obj_norm;
     else {
@@ -703,6 +846,11 @@ static void check_termination(orc_solver_t *s) {
         s->reason = TERMINATION_REASON_OPTIMAL;
         return;
     }
+    if (s->has_cones) { /* ray certificates are box-only (reference src/utils.cu:263-278) */
+        if (s->total_count >= c->iteration_limit) s->reason = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (s->cum_time >= c->time_sec_limit) s->reason = TERMINATION_REASON_TIME_LIMIT;
+        return;
+    }
     if (s->dray_obj > 0.0 && s->max_dray / s->dray_obj <= c->eps_infeasible) {
         s->reason = TERMINATION_REASON_PRIMAL_INFEASIBLE;
         return;
@@ -738,7 +886,7 @@ int orc_solver_run(orc_solver_t *s, int iters) {
         int tc = s->total_count;
         if (s->is_major || tc == 0 || tc % print_freq(tc) == 0) {
             residual(s);
-            if (s->is_major && tc < 3 * ef) infeasibility(s);
+            if (!s->has_cones && s->is_major && tc < 3 * ef) infeasibility(s);
             s->cum_time = elapsed(s);
             check_termination(s);
             if (s->reason != TERMINATION_REASON_UNSPECIFIED) break;
@@ -827,7 +975,8 @@ void orc_solver_free(orc_solver_t *s) {
     free_dummy(s->dummy);
     double *arr[] = {s->x0, s->x, s->xp, s->xr, s->y0, s->y, s->yp, s->yr, s->aty, s->ax, s->dslack, s->pslack,
                      s->pres, s->dres, s->dx, s->dy, s->qx, s->rx, s->rxbuf, s->cl_fin, s->cu_fin, s->lb_fin,
-                     s->ub_fin, s->grad, s->dirn, s->pbuf, s->diag_h, s->mdiag, s->minv};
+                     s->ub_fin, s->grad, s->dirn, s->pbuf, s->diag_h, s->mdiag, s->minv,
+                     s->proj_warm, s->res_warm, s->eff, s->snap, s->pgp};
     for (size_t i = 0; i < sizeof(arr) / sizeof(arr[0]); ++i) free(arr[i]);
     free(s);
 }
