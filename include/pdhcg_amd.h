@@ -55,6 +55,16 @@ void pdhcg_solver_free(pdhcg_solver_t *solver);
 int pdhcg_solver_timing(pdhcg_solver_t *solver, int enable);
 int pdhcg_solver_kernel_time(pdhcg_solver_t *solver, int kernel, double *total_ms, int *launches);
 
+/* The device cone operators the solver applies to its iterates, on host
+ * vectors of n entries (the reference applies them internally:
+ * src/cone_dispatch.cu:38-57 projection, :88-115 dual residual).
+ * d is the per-slot scaling (NULL: all ones); the projection is onto
+ * diag(d) K for every block of `cones`, in place on x.  The dual residual
+ * writes (rc - P_{K*}(rc)) * d on the cone slots of out and leaves the other
+ * slots of out as they are.  Return 0, or -1 with a message on stderr. */
+int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n);
+int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,8 @@ import ctypes as C
 import os
 from typing import Optional
 
+import numpy as np
+
 from . import capi
 from .problem import QPData, build, default_params, solve as _solve
 
@@ -48,6 +50,15 @@ def lib() -> C.CDLL:
         L.pdhcg_get_precision.restype = C.c_int
         L.pdhcg_request_cancel.restype = None
         L.pdhcg_request_cancel.argtypes = [C.c_int]
+        L.pdhcg_solver_timing.restype = C.c_int
+        L.pdhcg_solver_timing.argtypes = [C.c_void_p, C.c_int]
+        L.pdhcg_solver_kernel_time.restype = C.c_int
+        L.pdhcg_solver_kernel_time.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_int)]
+        L.pdhcg_project_cones.restype = C.c_int
+        L.pdhcg_project_cones.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p, C.c_int]
+        L.pdhcg_cone_dual_residual.restype = C.c_int
+        L.pdhcg_cone_dual_residual.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p,
+                                               capi.c_double_p, C.c_int]
         _lib = L
     return _lib
 
@@ -57,6 +68,30 @@ def solve(p: QPData, precision: int = PRECISION_FP64, **params) -> Optional[capi
     L = lib()
     L.pdhcg_set_precision(precision)
     return _solve(L, "", p, default_params(L, "", **params))
+
+
+def project_cones(specs, x, d=None) -> np.ndarray:
+    """Device projection of x onto diag(d) K for the blocks in ``specs``
+    ((type, start, v_dim[, alpha]) tuples); returns a new array."""
+    L = lib()
+    out = capi.carray(x).copy()
+    dd = None if d is None else capi.carray(d)
+    b, _keep = capi.cone_blocks(specs)
+    if L.pdhcg_project_cones(C.byref(b), capi.as_ptr(dd), capi.as_ptr(out), len(out)) != 0:
+        raise RuntimeError("pdhcg_project_cones failed")
+    return out
+
+
+def cone_dual_residual(specs, rc, d=None) -> np.ndarray:
+    """Device dual-cone residual (rc - P_{K*}(rc)) d on the cone slots (zero elsewhere)."""
+    L = lib()
+    r = capi.carray(rc)
+    out = np.zeros_like(r)
+    dd = None if d is None else capi.carray(d)
+    b, _keep = capi.cone_blocks(specs)
+    if L.pdhcg_cone_dual_residual(C.byref(b), capi.as_ptr(dd), capi.as_ptr(r), capi.as_ptr(out), len(r)) != 0:
+        raise RuntimeError("pdhcg_cone_dual_residual failed")
+    return out
 
 
 class Stepper:

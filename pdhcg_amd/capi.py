@@ -277,6 +277,26 @@ def cone_specs(specs) -> Any:
     return arr
 
 
+def cone_blocks(specs):
+    """Builds a cone_blocks_t from (type, start, v_dim[, alpha]) tuples.
+
+    Returns (struct, keep) where keep holds the arrays the struct points to."""
+    k = len(specs)
+    start = (C.c_int * max(1, k))(*[int(s[1]) for s in specs])
+    vdim = (C.c_int * max(1, k))(*[int(s[2]) for s in specs])
+    typ = (C.c_int * max(1, k))(*[int(s[0]) for s in specs])
+    alpha = (C.c_double * max(1, k))(*[float(s[3]) if len(s) > 3 else 0.0 for s in specs])
+    b = cone_blocks_t()
+    b.num_cones = k
+    b.start_idx = C.cast(start, c_int_p)
+    b.v_dim = C.cast(vdim, c_int_p)
+    b.type = C.cast(typ, c_int_p)
+    b.power_alpha = C.cast(alpha, c_double_p)
+    b.fixed_mask_size = 0
+    b.is_fixed = None
+    return b, (start, vdim, typ, alpha)
+
+
 @dataclass
 class Solution:
     status: str

@@ -1,0 +1,597 @@
+// Cone projections and dual-cone residuals on the device (gfx950), applied
+// in the scaled space: the projection is onto D K where d is the block's
+// variable scaling.  Algorithms follow the reference kernels
+//   SOC  reference src/kernels/pdhcg_soc_cone_kernels.cu:267-469 (projection),
+//        :471-726 (dual residual): closed form for uniform block scaling,
+//        bisection on the multiplier otherwise (60 steps, width 1e-13, warm start)
+//   RSOC reference src/kernels/pdhcg_rsoc_cone_kernels.cu:466-945 (rotated form)
+//   EXP  reference src/kernels/pdhcg_exp_cone_kernels.cu:26-131,:526-614
+//        (Newton on rho from 0, steps clamped to +-10, and its fallback)
+//   POW  reference src/kernels/pdhcg_power_cone_kernels.cu:25-132,:435-669
+//        (80-step bisection in log rho after homogeneity normalisation)
+//   PSD  reference src/kernels/pdhcg_psd_cone_kernels.cu:65-136 (svec <->
+//        dense, eigen-decomposition, positive part); the eigen-decomposition
+//        here is a cyclic two-sided Jacobi run by one workgroup per matrix.
+// All cone arithmetic is FP64, whatever the storage type X of the vectors.
+#pragma once
+
+#include <cfloat>
+
+#include "dev_common.hpp"
+
+namespace pdhcg {
+
+constexpr double kInvSqrt2 = 0.7071067811865475;
+constexpr double kSqrt2 = 1.41421356237309504880;
+
+// One list of cone blocks of a single type on the device.
+struct ConeList {
+    int count = 0;
+    int *start = nullptr;   // first slot of each block
+    int *vdim = nullptr;    // SOC/RSOC vector dimension, PSD order
+    double *alpha = nullptr;// power exponent
+    double *warm = nullptr; // projection warm starts (one per block)
+    double *rwarm = nullptr;// dual-residual warm starts
+};
+
+// ---------------- second-order cones ----------------
+
+template <class Get>
+__device__ double soc_multiplier_f(Get dh, const double *pv, int k, double w, double dw, double z, double lam) {
+    double sum = 0.0;
+    for (int m = 0; m < k; ++m) {
+        const double h = dh(m);
+        const double t = pv[m] * h / (h * h + 2.0 * lam);
+        sum += t * t;
+    }
+    const double tw = w * dw / (dw * dw + 2.0 * lam);
+    sum += tw * tw;
+    const double zt = z / (1.0 - 2.0 * lam);
+    return sum - zt * zt;
+}
+
+// Bisection on the multiplier of the weighted SOC projection.
+template <class Get>
+__device__ double soc_multiplier(Get dh, const double *pv, int k, double w, double dw, double z, double *warm) {
+    double lo, hi;
+    const bool zpos = z > 0.0;
+    if (zpos) {
+        lo = 0.0;
+        hi = 0.5 - 1e-14;
+    } else {
+        lo = 0.5 + 1e-14;
+        hi = 1.0;
+        for (int it = 0; it < 60; ++it) {
+            if (soc_multiplier_f(dh, pv, k, w, dw, z, hi) > 0.0) break;
+            lo = hi;
+            hi *= 2.0;
+        }
+    }
+    const double wl = *warm;
+    if (wl > lo && wl < hi) {
+        const double f = soc_multiplier_f(dh, pv, k, w, dw, z, wl);
+        if (fabs(f) < 1e-12) return wl;
+        if (zpos == (f > 0.0)) lo = wl;
+        else hi = wl;
+    }
+    for (int it = 0; it < 60; ++it) {
+        const double lam = 0.5 * (lo + hi);
+        const double f = soc_multiplier_f(dh, pv, k, w, dw, z, lam);
+        if (zpos == (f > 0.0)) lo = lam;
+        else hi = lam;
+        if ((hi - lo) / (1.0 + hi + lo) < 1e-13) break;
+    }
+    const double lam = 0.5 * (lo + hi);
+    *warm = lam;
+    return lam;
+}
+
+// Projects slots (v[0..k), a, b) held in FP64 registers/scratch `pv` (length
+// k + 2) in place.  rotated: (a, b) = (s, t), else (w, z).  d: block scaling.
+__device__ void soc_project_block(double *pv, int k, const double *d, bool rotated, double *warm) {
+    const double a = pv[k], b = pv[k + 1];
+    const double w = rotated ? (a - b) * kInvSqrt2 : a;
+    const double z = rotated ? (a + b) * kInvSqrt2 : b;
+    const double dref = rotated ? sqrt(d[k] * d[k + 1]) : d[k + 1];
+    const double dw = rotated ? 1.0 : d[k] / dref;
+    bool uniform = rotated || dw == 1.0;
+    for (int m = 0; m < k && uniform; ++m) uniform = d[m] == dref;
+    double wn, zn;
+    if (uniform) {
+        double ss = w * w;
+        for (int m = 0; m < k; ++m) ss += pv[m] * pv[m];
+        const double r = sqrt(ss);
+        if (r <= z) return;
+        if (r <= -z) {
+            for (int m = 0; m < k + 2; ++m) pv[m] = 0.0;
+            return;
+        }
+        const double sc = (z + r) / (2.0 * r);
+        for (int m = 0; m < k; ++m) pv[m] *= sc;
+        wn = sc * w;
+        zn = sc * r;
+    } else {
+        auto dh = [&](int m) { return d[m] / dref; };
+        double ri = (w / dw) * (w / dw), rp = (w * dw) * (w * dw);
+        for (int m = 0; m < k; ++m) {
+            const double h = dh(m);
+            ri += (pv[m] / h) * (pv[m] / h);
+            rp += (pv[m] * h) * (pv[m] * h);
+        }
+        if (sqrt(ri) <= z) return;
+        if (sqrt(rp) <= -z) {
+            for (int m = 0; m < k + 2; ++m) pv[m] = 0.0;
+            return;
+        }
+        const double lam = soc_multiplier(dh, pv, k, w, dw, z, warm);
+        for (int m = 0; m < k; ++m) {
+            const double h = dh(m);
+            pv[m] = pv[m] * h * h / (h * h + 2.0 * lam);
+        }
+        wn = rotated ? w / (1.0 + 2.0 * lam) : w * dw * dw / (dw * dw + 2.0 * lam);
+        zn = z / (1.0 - 2.0 * lam);
+    }
+    if (rotated) {
+        pv[k] = (zn + wn) * kInvSqrt2;
+        pv[k + 1] = (zn - wn) * kInvSqrt2;
+    } else {
+        pv[k] = wn;
+        pv[k + 1] = zn;
+    }
+}
+
+// Dual residual (rc - P_{K*}(rc)) d of a SOC/RSOC block; rc in `pr`
+// (length k + 2), result written to `out` (FP64).
+__device__ void soc_dual_residual_block(const double *pr, int k, const double *d, bool rotated, double *warm,
+                                        double *out) {
+    const double a = pr[k], b = pr[k + 1];
+    const double rw = rotated ? (a - b) * kInvSqrt2 : a;
+    const double rz = rotated ? (a + b) * kInvSqrt2 : b;
+    const double dref = rotated ? sqrt(d[k] * d[k + 1]) : d[k + 1];
+    const double ew = rotated ? 1.0 : dref / d[k];
+    bool uniform = rotated || ew == 1.0;
+    for (int m = 0; m < k && uniform; ++m) uniform = d[m] == dref;
+    double pa, pb;  // projected (a, b) slots
+    if (uniform) {
+        double ss = rw * rw;
+        for (int m = 0; m < k; ++m) ss += pr[m] * pr[m];
+        const double r = sqrt(ss);
+        double vf, pw, pz;
+        if (r <= rz) {
+            vf = 0.0;
+            pw = rw;
+            pz = rz;
+        } else if (r <= -rz) {
+            vf = 1.0;
+            pw = 0.0;
+            pz = 0.0;
+        } else {
+            const double sc = (rz + r) / (2.0 * r);
+            vf = 1.0 - sc;
+            pw = sc * rw;
+            pz = sc * r;
+        }
+        for (int m = 0; m < k; ++m) out[m] = pr[m] * vf * d[m];
+        if (rotated) {
+            if (r <= rz) {
+                pa = a;
+                pb = b;
+            } else {
+                pa = (pz + pw) * kInvSqrt2;
+                pb = (pz - pw) * kInvSqrt2;
+            }
+        } else {
+            pa = pw;
+            pb = pz;
+        }
+        out[k] = (a - pa) * d[k];
+        out[k + 1] = (b - pb) * d[k + 1];
+        return;
+    }
+    auto e = [&](int m) { return dref / d[m]; };
+    double ri = (rw / ew) * (rw / ew), rp = (rw * ew) * (rw * ew);
+    for (int m = 0; m < k; ++m) {
+        const double h = e(m);
+        ri += (pr[m] / h) * (pr[m] / h);
+        rp += (pr[m] * h) * (pr[m] * h);
+    }
+    if (sqrt(ri) <= rz) {
+        for (int m = 0; m < k + 2; ++m) out[m] = 0.0;
+        return;
+    }
+    if (sqrt(rp) <= -rz) {
+        for (int m = 0; m < k + 2; ++m) out[m] = pr[m] * d[m];
+        return;
+    }
+    const double lam = soc_multiplier(e, pr, k, rw, ew, rz, warm);
+    for (int m = 0; m < k; ++m) {
+        const double h = e(m);
+        out[m] = (pr[m] - pr[m] * h * h / (h * h + 2.0 * lam)) * d[m];
+    }
+    const double zn = rz / (1.0 - 2.0 * lam);
+    const double wn = rotated ? rw / (1.0 + 2.0 * lam) : rw * ew * ew / (ew * ew + 2.0 * lam);
+    if (rotated) {
+        pa = (zn + wn) * kInvSqrt2;
+        pb = (zn - wn) * kInvSqrt2;
+    } else {
+        pa = wn;
+        pb = zn;
+    }
+    out[k] = (a - pa) * d[k];
+    out[k + 1] = (b - pb) * d[k + 1];
+}
+
+// ---------------- exponential cone ----------------
+
+__device__ void exp_project_point(double r1, double r2, double r3, double d1, double d2, double d3, double *o) {
+    const double E = 2.718281828459045;
+    const double a1 = r1 / d1, a2 = r2 / d2, a3 = r3 / d3;
+    if (a2 > 0.0) {
+        const double ratio = a1 / a2;
+        if (ratio < 700.0 && a2 * exp(ratio) <= a3) {
+            o[0] = r1; o[1] = r2; o[2] = r3;
+            return;
+        }
+    } else if (a2 == 0.0 && a1 <= 0.0 && a3 >= 0.0) {
+        o[0] = r1; o[1] = r2; o[2] = r3;
+        return;
+    }
+    if (r1 > 0.0) {
+        const double ratio = (d2 * r2) / (d1 * r1);
+        if (ratio < 700.0 && d1 * r1 * exp(ratio) + E * d3 * r3 <= 0.0) {
+            o[0] = o[1] = o[2] = 0.0;
+            return;
+        }
+    } else if (r1 == 0.0 && r2 <= 0.0 && r3 <= 0.0) {
+        o[0] = o[1] = o[2] = 0.0;
+        return;
+    }
+    if (a1 <= 0.0 && a2 <= 0.0) {
+        o[0] = r1;
+        o[1] = 0.0;
+        o[2] = a3 < 0.0 ? 0.0 : r3;
+        return;
+    }
+    const double al = (d3 / d1) * (d3 / d1), be = (d3 / d2) * (d3 / d2);
+    double rho = 0.0;
+    bool diverged = false;
+    for (int it = 0; it < 100; ++it) {
+        if (rho >= 299.0 || rho <= -299.0) {
+            diverged = true;
+            break;
+        }
+        const double er = exp(rho), e2 = er * er, om = 1.0 - rho;
+        const double at = al - be * rho * om;
+        const double bt = be * a1 * om - al * a2;
+        const double f = a1 - a2 * rho + a3 * er * at + e2 * bt;
+        const double df = -a2 + a3 * er * (at + (-be * (1.0 - 2.0 * rho))) + e2 * (2.0 * bt + (-be * a1));
+        if (fabs(df) < 1e-300) break;
+        double st = f / df;
+        st = fmin(10.0, fmax(-10.0, st));
+        rho -= st;
+        if (fabs(st) < 1e-13 * (1.0 + fabs(rho))) break;
+    }
+    const double er = exp(rho);
+    const double u2 = (a1 + al * a3 * er) / (rho + al * er * er);
+    if (diverged || !isfinite(u2) || u2 <= 0.0) {
+        o[0] = r1 < 0.0 ? r1 : 0.0;
+        o[1] = 0.0;
+        o[2] = r3 > 0.0 ? r3 : 0.0;
+        return;
+    }
+    o[0] = d1 * (rho * u2);
+    o[1] = d2 * u2;
+    o[2] = d3 * (u2 * er);
+}
+
+// ---------------- power cone ----------------
+
+__device__ __forceinline__ double positive_root(double r, double q) {
+    if (!(q > 0.0)) return fmax(r, 0.0);
+    const double disc = hypot(r, 2.0 * sqrt(q));
+    return r >= 0.0 ? 0.5 * (r + disc) : (2.0 * q) / (disc - r);
+}
+
+__device__ void power_project_normalized(double rx, double ry, double rz, double wx, double wy, double wz, double al,
+                                         double *o) {
+    const double az = fabs(rz), sg = rz >= 0.0 ? 1.0 : -1.0, om = 1.0 - al;
+    if (az == 0.0) {
+        o[0] = fmax(rx, 0.0); o[1] = fmax(ry, 0.0); o[2] = 0.0;
+        return;
+    }
+    if (rx > 0.0 && ry > 0.0 && al * log(rx) + om * log(ry) >= log(az)) {
+        o[0] = rx; o[1] = ry; o[2] = rz;
+        return;
+    }
+    if (rx <= 0.0 && ry <= 0.0) {
+        const double u = rx < 0.0 ? (-wx * rx) / al : 0.0;
+        const double v = ry < 0.0 ? (-wy * ry) / om : 0.0;
+        if (u > 0.0 && v > 0.0 && al * log(u) + om * log(v) >= log(wz) + log(az)) {
+            o[0] = o[1] = o[2] = 0.0;
+            return;
+        }
+    }
+    const double cx = 4.0 * al * (wz / wx), cy = 4.0 * om * (wz / wy);
+    double lo = log(DBL_MIN), hi = log(az);
+    if (!(hi > lo)) {
+        o[0] = fmax(rx, 0.0); o[1] = fmax(ry, 0.0); o[2] = 0.0;
+        return;
+    }
+    for (int it = 0; it < 80; ++it) {
+        const double lr = lo + 0.5 * (hi - lo), rho = exp(lr);
+        const double x = positive_root(rx, 0.25 * cx * rho * (az - rho));
+        const double y = positive_root(ry, 0.25 * cy * rho * (az - rho));
+        if (x > 0.0 && y > 0.0 && al * log(x) + om * log(y) > lr) lo = lr;
+        else hi = lr;
+    }
+    const double rho = exp(lo + 0.5 * (hi - lo));
+    o[0] = positive_root(rx, 0.25 * cx * rho * (az - rho));
+    o[1] = positive_root(ry, 0.25 * cy * rho * (az - rho));
+    o[2] = sg * fmin(rho, exp(al * log(o[0]) + om * log(o[1])));
+}
+
+__device__ void power_project_point(double rx, double ry, double rz, double wx, double wy, double wz, double al,
+                                    double *o) {
+    const double sc = fmax(fabs(rx), fmax(fabs(ry), fabs(rz)));
+    if (!(sc > 0.0) || !isfinite(sc)) {
+        power_project_normalized(rx, ry, rz, wx, wy, wz, al, o);
+        return;
+    }
+    power_project_normalized(rx / sc, ry / sc, rz / sc, wx, wy, wz, al, o);
+    o[0] *= sc; o[1] *= sc; o[2] *= sc;
+}
+
+// ---------------- per-block kernels (one thread per block) ----------------
+
+constexpr int kMaxThreadCone = 64;  // longest SOC/RSOC block handled in registers/stack
+
+// type: 0 rotated, 1 standard SOC, 2 exponential, 3 power (cone_type_t values)
+template <class X, int Type>
+__global__ void __launch_bounds__(kBlock) cone_project_kernel(X *v, const double *d, ConeList L, int use_rwarm,
+                                                               const int *gate) {
+    if (gate && *gate == 0) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= L.count) return;
+    const int s = L.start[b];
+    double *warm = use_rwarm ? L.rwarm + b : L.warm + b;
+    if (Type == CONE_STANDARD_SOC || Type == CONE_ROTATED_SOC) {
+        const int k = L.vdim[b];
+        double pv[kMaxThreadCone + 2];
+        for (int m = 0; m < k + 2; ++m) pv[m] = (double)v[s + m];
+        soc_project_block(pv, k, d + s, Type == CONE_ROTATED_SOC, warm);
+        for (int m = 0; m < k + 2; ++m) v[s + m] = (X)pv[m];
+    } else if (Type == CONE_EXPONENTIAL) {
+        double o[3];
+        exp_project_point((double)v[s], (double)v[s + 1], (double)v[s + 2], d[s], d[s + 1], d[s + 2], o);
+        v[s] = (X)o[0]; v[s + 1] = (X)o[1]; v[s + 2] = (X)o[2];
+    } else {
+        double o[3];
+        const double d0 = d[s], d1 = d[s + 1], d2 = d[s + 2];
+        power_project_point((double)v[s] / d0, (double)v[s + 1] / d1, (double)v[s + 2] / d2, d0 * d0, d1 * d1, d2 * d2,
+                            L.alpha[b], o);
+        v[s] = (X)(o[0] * d0); v[s + 1] = (X)(o[1] * d1); v[s + 2] = (X)(o[2] * d2);
+    }
+}
+
+// Dual-cone residual of rc = c_eff - A'y on one block, written into dres
+// (reference src/cone_dispatch.cu:88-115).  c_eff = c + qx when qx != null.
+template <class X, int Type>
+__global__ void __launch_bounds__(kBlock) cone_dual_residual_kernel(double *dres, const X *c, const X *qx,
+                                                                     const X *aty, const double *d, ConeList L) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= L.count) return;
+    const int s = L.start[b];
+    auto rc = [&](int i) { return (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i]; };
+    if (Type == CONE_STANDARD_SOC || Type == CONE_ROTATED_SOC) {
+        const int k = L.vdim[b];
+        double pr[kMaxThreadCone + 2], out[kMaxThreadCone + 2];
+        for (int m = 0; m < k + 2; ++m) pr[m] = rc(s + m);
+        soc_dual_residual_block(pr, k, d + s, Type == CONE_ROTATED_SOC, L.rwarm + b, out);
+        for (int m = 0; m < k + 2; ++m) dres[s + m] = out[m];
+    } else if (Type == CONE_EXPONENTIAL) {
+        double o[3];
+        exp_project_point(-rc(s), -rc(s + 1), -rc(s + 2), 1.0 / d[s], 1.0 / d[s + 1], 1.0 / d[s + 2], o);
+        dres[s] = -o[0] * d[s];
+        dres[s + 1] = -o[1] * d[s + 1];
+        dres[s + 2] = -o[2] * d[s + 2];
+    } else {
+        double o[3];
+        power_project_point(-rc(s) * d[s], -rc(s + 1) * d[s + 1], -rc(s + 2) * d[s + 2], 1.0, 1.0, 1.0, L.alpha[b], o);
+        dres[s] = -o[0];
+        dres[s + 1] = -o[1];
+        dres[s + 2] = -o[2];
+    }
+}
+
+// Power-cone primal violation (reference pdhcg_power_cone_kernels.cu:621-669),
+// reduced over blocks: out = [sum abs^2, sum rel^2, max abs, max rel].
+template <class X>
+__global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, const double *d, ConeList L,
+                                                                  double hscale, GridRed gr) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < L.count; b += gridDim.x * blockDim.x) {
+        const int s = L.start[b];
+        const double x = (double)xp[s] / d[s], y = (double)xp[s + 1] / d[s + 1], z = (double)xp[s + 2] / d[s + 2];
+        double av, rv;
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) {
+            av = rv = INFINITY;
+        } else {
+            double viol = fmax(-x, -y);
+            const double az = fabs(z);
+            bool done = false;
+            if (az > 0.0) {
+                double bound = 0.0;
+                if (x > 0.0 && y > 0.0) {
+                    const double al = L.alpha[b];
+                    const double lb = al * log(x) + (1.0 - al) * log(y), lz = log(az);
+                    if (lb + 64.0 * DBL_EPSILON * (1.0 + fabs(lb) + fabs(lz)) >= lz) {
+                        viol = fmax(viol, 0.0);
+                        done = true;
+                    } else {
+                        bound = exp(lb);
+                    }
+                }
+                if (!done) viol = fmax(viol, az - bound);
+            }
+            viol = fmax(viol, 0.0);
+            av = viol;
+            rv = viol / (hscale + fmax(fabs(x), fmax(fabs(y), az)));
+        }
+        v[0] += av * av;
+        v[1] += rv * rv;
+        v[2] = fmax(v[2], av);
+        v[3] = fmax(v[3], rv);
+    }
+    grid_reduce<2, 2>(gr, v);
+}
+
+// ---------------- PSD cone: one workgroup per matrix ----------------
+
+// Projects the svec block of order k at v + start onto the PSD cone.  The
+// dense matrix and the rotation accumulator live in `work` (2 k^2 doubles per
+// matrix).  Cyclic Jacobi: every (p, q) pair in row order, the rotation
+// applied to rows / columns by the whole workgroup; sweeps until the
+// off-diagonal mass is below 1e-30 of the total (or 100 sweeps).
+template <class X>
+__global__ void __launch_bounds__(kBlock) psd_project_kernel(X *v, const int *starts, int k, double *work,
+                                                              const int *gate) {
+    if (gate && *gate == 0) return;
+    const int mat = blockIdx.x;
+    const int start = starts[mat];
+    double *A = work + (size_t)mat * 2 * k * k;
+    double *V = A + (size_t)k * k;
+    __shared__ double cs[2];
+    __shared__ double red[kBlock / kWave];
+    __shared__ int stop;
+    const int L = k * (k + 1) / 2;
+    if (k == 1) {
+        if (threadIdx.x == 0) v[start] = (X)fmax((double)v[start], 0.0);
+        return;
+    }
+    // svec -> dense (lower triangle, column-major, off-diagonal / sqrt 2)
+    for (int slot = threadIdx.x; slot < L; slot += blockDim.x) {
+        int c = 0, rem = slot;
+        while (rem >= k - c) {
+            rem -= k - c;
+            ++c;
+        }
+        const int r = c + rem;
+        double val = (double)v[start + slot];
+        if (r != c) val *= kInvSqrt2;
+        A[r * k + c] = val;
+        A[c * k + r] = val;
+    }
+    for (int i = threadIdx.x; i < k * k; i += blockDim.x) V[i] = (i / k == i % k) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = threadIdx.x; i < k * k; i += blockDim.x) {
+            const double a = A[i];
+            tot += a * a;
+            if (i / k != i % k) off += a * a;
+        }
+        off = wave_sum(off);
+        tot = wave_sum(tot);
+        if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = off;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double o = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) o += red[w];
+            red[0] = o;
+        }
+        __syncthreads();
+        const double off_total = red[0];
+        __syncthreads();
+        if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+            stop = (off_total <= 1e-30 * t || off_total == 0.0) ? 1 : 0;
+        }
+        __syncthreads();
+        if (stop) break;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                if (threadIdx.x == 0) {
+                    const double apq = A[p * k + q];
+                    if (apq == 0.0) {
+                        cs[0] = 1.0;
+                        cs[1] = 0.0;
+                    } else {
+                        const double theta = (A[q * k + q] - A[p * k + p]) / (2.0 * apq);
+                        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        const double c = 1.0 / sqrt(t * t + 1.0);
+                        cs[0] = c;
+                        cs[1] = t * c;
+                    }
+                }
+                __syncthreads();
+                const double c = cs[0], s = cs[1];
+                if (s != 0.0) {
+                    for (int r = threadIdx.x; r < k; r += blockDim.x) {
+                        const double arp = A[r * k + p], arq = A[r * k + q];
+                        A[r * k + p] = c * arp - s * arq;
+                        A[r * k + q] = s * arp + c * arq;
+                        const double vrp = V[r * k + p], vrq = V[r * k + q];
+                        V[r * k + p] = c * vrp - s * vrq;
+                        V[r * k + q] = s * vrp + c * vrq;
+                    }
+                    __syncthreads();
+                    for (int r = threadIdx.x; r < k; r += blockDim.x) {
+                        const double apr = A[p * k + r], aqr = A[q * k + r];
+                        A[p * k + r] = c * apr - s * aqr;
+                        A[q * k + r] = s * apr + c * aqr;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    // svec of V max(diag(A), 0) V'
+    for (int slot = threadIdx.x; slot < L; slot += blockDim.x) {
+        int c = 0, rem = slot;
+        while (rem >= k - c) {
+            rem -= k - c;
+            ++c;
+        }
+        const int r = c + rem;
+        double val = 0.0;
+        for (int e = 0; e < k; ++e) val += fmax(A[e * k + e], 0.0) * V[r * k + e] * V[c * k + e];
+        if (r != c) val *= kSqrt2;
+        v[start + slot] = (X)val;
+    }
+}
+
+// Elementwise helpers on cone slots, driven by a slot mask (kMaskCone on
+// SOC/RSOC/EXP/POWER slots, kMaskPsd on PSD slots, 0 elsewhere).
+constexpr char kMaskCone = 1;
+constexpr char kMaskPsd = 2;
+
+template <class X>
+__global__ void __launch_bounds__(kBlock) cone_reflect_kernel(X *xr, const X *xp, const X *x, const char *mask, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mask[i]) xr[i] = (X)(2.0 * (double)xp[i] - (double)x[i]);
+}
+
+// PSD dual residual, first half: rc = c_eff - A'y on PSD slots, kept in dres
+// and copied into p for the in-place projection.
+template <class X>
+__global__ void __launch_bounds__(kBlock) psd_rc_kernel(X *p, double *dres, const X *c, const X *qx, const X *aty,
+                                                         const char *mask, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mask[i] == kMaskPsd) {
+            const double rc = (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i];
+            dres[i] = rc;
+            p[i] = (X)rc;
+        }
+}
+
+// Second half: dres = (rc - P_psd(rc)) d (the PSD cone is self-dual).
+template <class X>
+__global__ void __launch_bounds__(kBlock) psd_dres_kernel(double *dres, const X *p, const double *d, const char *mask,
+                                                           int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mask[i] == kMaskPsd) dres[i] = (dres[i] - (double)p[i]) * d[i];
+}
+
+}  // namespace pdhcg

@@ -11,7 +11,7 @@ namespace pdhcg {
 
 // Scalars that live on the device between launches (FP64 throughout).
 struct DevScalars {
-    double red[kMaxRedValues];  // generic reduction results
+    double red[32];             // generic reduction results (several launches write disjoint ranges)
     double bb_nrm;              // sqrt(s's) or sqrt(s'Ms) of the current BB step
     double bb_stms;             // s'Ms (Jacobi-preconditioned BB)
     double bb_tmp;              // s'(g_new - g_old)
@@ -190,7 +190,8 @@ template <class X>
 __global__ void __launch_bounds__(kBlock) bb_init_kernel(const X *aty, X *grad, X *dirn, const X *x, X *xp,
                                                           const X *c, const X *qx, const X *lb, const X *ub,
                                                           const X *m_inv, const X *m_diag, double alpha0, int n,
-                                                          GridRed g, DevScalars *ds, double tol, int limit) {
+                                                          GridRed g, DevScalars *ds, double tol, int limit,
+                                                          int defer) {
     const bool pre = m_inv != nullptr;
     const double a0 = pre ? alpha0 * (ds->tol_scale * ds->tol_scale) : alpha0;
     double v[1] = {0.0};
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) bb_init_kernel(const X *aty, X *grad, 
         dirn[i] = (X)s;
         v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
     }
+    if (defer) return;  // cones: the step is re-measured after the projection
     if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
         ds->bb_iter = 1;
         if (pre) ds->bb_stms = v[0];
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(kBlock) bb_gradient_kernel(const X *xp, const 
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *grad, X *dirn, const X *lb, const X *ub,
                                                                const X *m_inv, const X *m_diag, int n, GridRed g,
-                                                               DevScalars *ds, double tol, int limit) {
+                                                               DevScalars *ds, double tol, int limit, int defer) {
     if (ds->bb_active == 0) return;
     const bool pre = m_inv != nullptr;
     const double a = ds->bb_alpha;
@@ -253,6 +255,7 @@ __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *gr
         xp[i] = (X)nx;
         v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
     }
+    if (defer) return;
     if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
         const int it = ds->bb_iter + 1;
         ds->bb_iter = it;
@@ -260,6 +263,79 @@ __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *gr
         ds->bb_nrm = sqrt(v[0]);
         ds->bb_active = (it < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
     }
+}
+
+// With cones: after the cone projection of x_p the step is s = x_p - base
+// (base = x_k for the first step, the pre-step snapshot afterwards) and the
+// stopping test uses that s (reference :561-566,652-657 and :575-597).
+template <class X>
+__global__ void __launch_bounds__(kBlock) bb_cone_step_kernel(const X *xp, const X *base, X *dirn, const X *m_diag,
+                                                               int n, int first, GridRed g, DevScalars *ds, double tol,
+                                                               int limit) {
+    if (!first && ds->bb_active == 0) return;
+    const bool pre = m_diag != nullptr;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double s = (double)(X)((double)xp[i] - (double)base[i]);
+        dirn[i] = (X)s;
+        v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
+    }
+    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
+        const int it = first ? 1 : ds->bb_iter + 1;
+        ds->bb_iter = it;
+        if (pre) ds->bb_stms = v[0];
+        ds->bb_nrm = sqrt(v[0]);
+        ds->bb_active = (it < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
+    }
+}
+
+// Projected-gradient point for the conic stationarity residual
+// (reference pdhcg_kernels.cu:75-91): p = P_box(x - t (c_eff - A'y)).
+template <class X>
+__global__ void __launch_bounds__(kBlock) projected_gradient_point_kernel(X *pgp, const X *xp, const X *c, const X *qx,
+                                                                           const X *aty, const X *lb, const X *ub,
+                                                                           double t, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double g = (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i];
+        pgp[i] = (X)fmax((double)lb[i], fmin((double)xp[i] - t * g, (double)ub[i]));
+    }
+}
+
+// dres = the larger-magnitude of dres and (x - p)/t D_c (reference
+// pdhcg_kernels.cu:93-109), then the dual-residual norms of the whole vector:
+// out = [sum dres^2, max |dres|].
+template <class X>
+__global__ void __launch_bounds__(kBlock) augment_dual_residual_kernel(double *dres, const X *xp, const X *pgp,
+                                                                        const double *var_rescale, double t, int n,
+                                                                        GridRed g) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double r = ((double)xp[i] - (double)pgp[i]) / t * var_rescale[i];
+        if (!isfinite(r)) r = copysign(INFINITY, r);
+        double dr = dres[i];
+        if (fabs(r) > fabs(dr)) dr = r;
+        dres[i] = dr;
+        v[0] += dr * dr;
+        v[1] = fmax(v[1], abs_or_inf(dr));
+    }
+    grid_reduce<1, 1>(g, v);
+}
+
+// Dual objective correction for cone slots: the cone dual slack is
+// c_eff - A'y (reference pdhcg_cone_common_kernels.cu:23-40), stored into
+// dslack there; out = [sum over cone slots of (c_eff - A'y - slack_box) x].
+template <class X>
+__global__ void __launch_bounds__(kBlock) cone_slack_correction_kernel(const X *c, const X *qx, const X *aty,
+                                                                        X *dslack, const X *xp, const char *mask,
+                                                                        int n, GridRed g) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mask[i]) {
+            const double cs = (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i];
+            v[0] += (cs - (double)dslack[i]) * (double)xp[i];
+            dslack[i] = (X)cs;
+        }
+    grid_reduce<1, 0>(g, v);
 }
 
 // x_bar = 2 x_p - x_k, inner count, optional Halpern (reference :664-668, kernel :577-589).
@@ -417,7 +493,7 @@ template <class X>
 __global__ void __launch_bounds__(kBlock) residual_cols_kernel(const X *c, const X *aty, const X *qx, const X *xp,
                                                                 const X *lb, const X *ub, X *dslack,
                                                                 const double *var_rescale, double tau, int qp, int n,
-                                                                GridRed gr) {
+                                                                double *dres_out, GridRed gr) {
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double x = (double)xp[i], a = (double)aty[i], ci = (double)c[i], dc = var_rescale[i];
@@ -434,6 +510,7 @@ __global__ void __launch_bounds__(kBlock) residual_cols_kernel(const X *c, const
             ds = (double)dslack[i];
             dres = (ci - a - ds) * dc;
         }
+        if (dres_out) dres_out[i] = dres;  // cones adjust it before the norms are taken
         v[0] += dres * dres;
         v[1] += ci * x;
         v[2] += x * q;

@@ -10,13 +10,16 @@
 // Solver<double, double> is the plain FP64 form; Solver<float, float> keeps
 // matrix values and vector traffic in FP32 (FP32 SpMV accumulation) with all
 // reductions, scalars, restart and KKT logic in FP64.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/pdhcg_amd.h"
+#include "dev_cones.hpp"
 #include "dev_kernels.hpp"
 #include "dev_timer.hpp"
 #include "host_scale.h"
@@ -84,6 +87,153 @@ struct RunStats {
     int inner_total = 0;
 };
 
+// The variable cone blocks on the device: one block list per type, PSD
+// blocks grouped by order, and a slot mask (reference
+// internal/internal_types.h:106-127, dispatch src/cone_dispatch.cu).
+struct PsdGroup {
+    int order = 0, count = 0;
+    int *starts = nullptr;
+    double *work = nullptr;  // 2 order^2 doubles per matrix
+};
+
+class DevConeSet {
+  public:
+    bool psd = false, power = false;
+    ConeList rsoc, soc, expc, powc;
+    std::vector<PsdGroup> groups;
+    char *mask = nullptr;
+
+    DevConeSet() = default;
+    DevConeSet(const DevConeSet &) = delete;
+    DevConeSet &operator=(const DevConeSet &) = delete;
+    ~DevConeSet() { release(); }
+
+    void build(const cone_blocks_t *b, int n) {
+        struct HostList {
+            std::vector<int> start, vdim;
+            std::vector<double> alpha;
+        } lists[4];  // rotated SOC, SOC, exponential, power
+        std::vector<char> hmask((size_t)n, 0);
+        std::vector<std::pair<int, int>> ps;  // (order, start)
+        for (int k = 0; k < b->num_cones; ++k) {
+            const int st = b->start_idx[k], vd = b->v_dim[k], len = cone_block_slots(b, k);
+            const cone_type_t t = b->type[k];
+            if (st < 0 || len < 0 || (long long)st + len > n) fail("cone block %d lies outside the variables", k);
+            if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone)
+                fail("second-order cone block %d has %d vector slots; at most %d are supported", k, vd,
+                     kMaxThreadCone);
+            for (int i = st; i < st + len; ++i) hmask[i] = t == CONE_PSD ? kMaskPsd : kMaskCone;
+            if (t == CONE_PSD) {
+                psd = true;
+                ps.emplace_back(vd, st);
+                continue;
+            }
+            if (t == CONE_POWER) power = true;
+            const int li = t == CONE_ROTATED_SOC ? 0 : t == CONE_STANDARD_SOC ? 1 : t == CONE_EXPONENTIAL ? 2 : 3;
+            lists[li].start.push_back(st);
+            lists[li].vdim.push_back(vd);
+            lists[li].alpha.push_back(t == CONE_POWER && b->power_alpha ? b->power_alpha[k] : 0.0);
+        }
+        ConeList *dst[4] = {&rsoc, &soc, &expc, &powc};
+        for (int li = 0; li < 4; ++li) {
+            const HostList &h = lists[li];
+            ConeList &L = *dst[li];
+            L.count = (int)h.start.size();
+            if (!L.count) continue;
+            L.start = dupload<int>(h.start, h.start.size());
+            L.vdim = dupload<int>(h.vdim, h.vdim.size());
+            L.alpha = dupload<double>(h.alpha, h.alpha.size());
+            L.warm = dzeros<double>(h.start.size());
+            L.rwarm = dzeros<double>(h.start.size());
+        }
+        std::sort(ps.begin(), ps.end());
+        for (size_t i = 0; i < ps.size();) {
+            size_t j = i;
+            std::vector<int> starts;
+            while (j < ps.size() && ps[j].first == ps[i].first) starts.push_back(ps[j++].second);
+            PsdGroup g;
+            g.order = ps[i].first;
+            g.count = (int)starts.size();
+            g.starts = dupload<int>(starts, starts.size());
+            g.work = dalloc<double>((size_t)2 * g.order * g.order * g.count);
+            groups.push_back(g);
+            i = j;
+        }
+        mask = dupload<char>(hmask, hmask.size());
+    }
+
+    // Projects v onto the scaled cones D K block by block (reference
+    // src/cone_dispatch.cu:38-57).  residual_warm selects the warm starts of
+    // the residual evaluation instead of those of the iteration.
+    template <class X>
+    void project(X *v, const double *d, bool residual_warm, const int *gate, hipStream_t st) const {
+        const int rw = residual_warm ? 1 : 0;
+        if (rsoc.count)
+            hipLaunchKernelGGL((cone_project_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0, st, v,
+                               d, rsoc, rw, gate);
+        if (soc.count)
+            hipLaunchKernelGGL((cone_project_kernel<X, CONE_STANDARD_SOC>), blocks(soc.count), dim3(kBlock), 0, st, v,
+                               d, soc, rw, gate);
+        if (expc.count)
+            hipLaunchKernelGGL((cone_project_kernel<X, CONE_EXPONENTIAL>), blocks(expc.count), dim3(kBlock), 0, st, v,
+                               d, expc, rw, gate);
+        if (powc.count)
+            hipLaunchKernelGGL((cone_project_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v, d,
+                               powc, rw, gate);
+        for (const PsdGroup &g : groups)
+            hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, v, g.starts, g.order, g.work,
+                               gate);
+    }
+
+    // Dual-cone residuals (rc - P_{K*}(rc)) d of rc = c + qx - aty (qx may be
+    // null) written over dres on every cone slot; scratch (n entries) holds the
+    // PSD projections (reference src/cone_dispatch.cu:88-115).
+    template <class X>
+    void dual_residual(double *dres, X *scratch, const X *c, const X *qx, const X *aty, const double *d, int n,
+                       hipStream_t st) const {
+        if (rsoc.count)
+            hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0,
+                               st, dres, c, qx, aty, d, rsoc);
+        if (soc.count)
+            hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_STANDARD_SOC>), blocks(soc.count), dim3(kBlock), 0,
+                               st, dres, c, qx, aty, d, soc);
+        if (expc.count)
+            hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_EXPONENTIAL>), blocks(expc.count), dim3(kBlock), 0,
+                               st, dres, c, qx, aty, d, expc);
+        if (powc.count)
+            hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st,
+                               dres, c, qx, aty, d, powc);
+        if (psd) {
+            hipLaunchKernelGGL(psd_rc_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, scratch, dres, c, qx, aty,
+                               mask, n);
+            for (const PsdGroup &g : groups)
+                hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, scratch, g.starts,
+                                   g.order, g.work, nullptr);
+            hipLaunchKernelGGL(psd_dres_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, dres, scratch, d, mask,
+                               n);
+        }
+    }
+
+    void release() {
+        for (ConeList *L : {&soc, &rsoc, &expc, &powc}) {
+            void *lp[] = {L->start, L->vdim, L->alpha, L->warm, L->rwarm};
+            for (void *p : lp)
+                if (p) (void)hipFree(p);
+            *L = ConeList();
+        }
+        for (PsdGroup &g : groups) {
+            if (g.starts) (void)hipFree(g.starts);
+            if (g.work) (void)hipFree(g.work);
+        }
+        groups.clear();
+        if (mask) (void)hipFree(mask);
+        mask = nullptr;
+    }
+
+  private:
+    static dim3 blocks(int count) { return dim3((unsigned)((count + kBlock - 1) / kBlock)); }
+};
+
 }  // namespace
 
 class SolverBase {
@@ -118,6 +268,7 @@ template <class V, class X> class Solver final : public SolverBase {
 
     // ---- setup helpers
     void upload(const ScaledProblem &sp);
+    void setup_cones(const ScaledProblem &sp, std::vector<double> &xs);
     void init_inner(const ScaledProblem &sp);
     double power_sv();
     double power_eig(bool minimum, double shift);
@@ -129,7 +280,12 @@ template <class V, class X> class Solver final : public SolverBase {
         HIP_OK(hipStreamSynchronize(st_));
     }
     int grid(long long n) const { return blocks_for(n); }
+    // LP and diagonal-Q primal steps are closed form; the rest use BB
+    bool closed_prox() const { return qtype_ == PDHCG_NON_Q || (qtype_ == PDHCG_DIAG_Q && !has_psd_); }
     void q_product(const X *x, X *qx, const int *gate);
+    void project_cones(X *v, bool residual_warm, const int *gate) {
+        cones_.project(v, var_rescale_, residual_warm, gate, st_);
+    }
     void bb_prox(double tau, bool fuse_halpern, double w);
     void pdhg_step(bool fuse_halpern);
     void halpern(double w);
@@ -168,6 +324,12 @@ template <class V, class X> class Solver final : public SolverBase {
     X *y0_ = nullptr, *y_ = nullptr, *yp_ = nullptr, *yr_ = nullptr;
     X *aty_ = nullptr, *ax_ = nullptr, *dslack_ = nullptr, *dx_ = nullptr, *dy_ = nullptr, *qx_ = nullptr;
 
+    // variable cone blocks, one list per type (reference internal/internal_types.h:106-127)
+    bool has_cones_ = false, has_psd_ = false, has_power_ = false;
+    DevConeSet cones_;
+    X *snap_ = nullptr, *pgp_ = nullptr;
+    double *dres_ = nullptr;
+
     // BB
     bool bb_ = false, precond_ = false;
     X *grad_ = nullptr, *dirn_ = nullptr, *mdiag_ = nullptr, *minv_ = nullptr;
@@ -197,6 +359,11 @@ template <class V, class X> class Solver final : public SolverBase {
 
 template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params) : P_(params) {
     const auto t0 = std::chrono::steady_clock::now();
+    if (sp.qtype == PDHCG_DIAG_Q && sp.cones && sp.cones->num_cones > 0) {
+        bool psd = false;
+        for (int k = 0; k < sp.cones->num_cones; ++k) psd |= sp.cones->type[k] == CONE_PSD;
+        if (!psd) fail("the diagonal-Q proximal step with cone blocks is not supported");
+    }
     HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     n_ = sp.n;
     m_ = sp.m;
@@ -321,11 +488,16 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         for (int i = 0; i < n_; ++i) xs[i] = sp.primal_start[i] * sp.var_rescale[i] * sp.cbr;
     if (!sp.dual_start.empty())
         for (int i = 0; i < m_; ++i) ys[i] = sp.dual_start[i] * sp.con_rescale[i] * sp.ovr;
-    for (int i = 0; i < n_; ++i) xs[i] = std::fmax(sp.lb[i], std::fmin(xs[i], sp.ub[i]));
+    setup_cones(sp, xs);
     x0_ = dupload<X>(xs, n_);
-    x_ = dupload<X>(xs, n_);
-    xp_ = dupload<X>(xs, n_);
-    xr_ = dupload<X>(xs, n_);
+    if (has_cones_) project_cones(x0_, false, nullptr);
+    hipLaunchKernelGGL(clamp_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, x0_, lb_, ub_, n_);
+    x_ = dalloc<X>(n_);
+    xp_ = dalloc<X>(n_);
+    xr_ = dalloc<X>(n_);
+    for (X *dst : {x_, xp_, xr_})
+        HIP_OK(hipMemcpyAsync(dst, x0_, (size_t)n_ * sizeof(X), hipMemcpyDeviceToDevice, st_));
+    HIP_OK(hipStreamSynchronize(st_));
     y0_ = dupload<X>(ys, m_);
     y_ = dupload<X>(ys, m_);
     yp_ = dupload<X>(ys, m_);
@@ -336,6 +508,29 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     dx_ = dzeros<X>(n_);
     dy_ = dzeros<X>(m_);
     qx_ = dzeros<X>(n_);
+}
+
+// Initial auxiliary values of SOC / RSOC blocks written into the
+// (unprojected) starting point (reference src/solver_state.cu:578-686), and
+// the device cone set with its work vectors.
+template <class V, class X> void Solver<V, X>::setup_cones(const ScaledProblem &sp, std::vector<double> &xs) {
+    const cone_blocks_t *b = sp.cones;
+    if (!b || b->num_cones <= 0) return;
+    cones_.build(b, n_);
+    for (int k = 0; k < b->num_cones; ++k) {
+        const int aux = b->start_idx[k] + b->v_dim[k];
+        if (b->type[k] == CONE_STANDARD_SOC) {
+            xs[aux] = -kInvSqrt2 * sp.cbr * sp.var_rescale[aux];
+            xs[(size_t)aux + 1] = kInvSqrt2 * sp.cbr * sp.var_rescale[(size_t)aux + 1];
+        } else if (b->type[k] == CONE_ROTATED_SOC) {
+            xs[(size_t)aux + 1] = sp.cbr * sp.var_rescale[(size_t)aux + 1];
+        }
+    }
+    has_cones_ = true;
+    has_psd_ = cones_.psd;
+    has_power_ = cones_.power;
+    pgp_ = dzeros<X>(n_);
+    dres_ = dzeros<double>(n_);
 }
 
 // Jacobi metric diag(Q) + diag(R'DR) from the scaled host data
@@ -350,13 +545,16 @@ template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &s
     if (t0 <= 0.0) t0 = 1e-3;
     if (tmin <= 0.0) tmin = 1e-9;
     inner_limit_ = lim;
-    if (qtype_ == PDHCG_NON_Q || qtype_ == PDHCG_DIAG_Q) return;
+    // the diagonal-Q prox is closed form unless a PSD block needs the BB solve
+    // (reference src/pdhg_core_op.cu:740-741)
+    if (qtype_ == PDHCG_NON_Q || (qtype_ == PDHCG_DIAG_Q && !has_psd_)) return;
     bb_ = true;
     inner_tol_ = t0;
     inner_min_tol_ = tmin;
     grad_ = dzeros<X>(n_);
     dirn_ = dzeros<X>(n_);
-    precond_ = P_.diag_jacobi_precond;
+    if (has_cones_) snap_ = dzeros<X>(n_);
+    precond_ = P_.diag_jacobi_precond && !has_psd_;
     if (!precond_) return;
     std::vector<double> dh((size_t)n_, 0.0);
     if (qtype_ == PDHCG_SPARSE_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
@@ -403,9 +601,10 @@ template <class V, class X> Solver<V, X>::~Solver() {
                     cu_fin_, lb_fin_, ub_fin_, con_rescale_, var_rescale_, x0_,     x_,           xp_,
                     xr_,     y0_,    y_,      yp_,     yr_,          aty_,        ax_,          dslack_,
                     dx_,     dy_,    qx_,     grad_,   dirn_,        mdiag_,      minv_,        diag_h_,
-                    red_partials_, red_counter_, ds_};
+                    red_partials_, red_counter_, ds_, snap_, pgp_, dres_};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    cones_.release();
     if (hs_) (void)hipHostFree(hs_);
     if (st_) (void)hipStreamDestroy(st_);
 }
@@ -552,8 +751,16 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     const X *mi = precond_ ? minv_ : nullptr;
     const X *md = precond_ ? mdiag_ : nullptr;
     q_product(x_, qx_, nullptr);
+    // With cones every box step is followed by the cone projection and the
+    // step is re-measured from the projected point (reference :561-566,652-657).
+    const int defer = has_cones_ ? 1 : 0;
     hipLaunchKernelGGL(bb_init_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, aty_, grad_, dirn_, x_, xp_, c_, qx_, lb_,
-                       ub_, mi, md, alpha0, n_, red(), ds_, inner_tol_, inner_limit_);
+                       ub_, mi, md, alpha0, n_, red(), ds_, inner_tol_, inner_limit_, defer);
+    if (has_cones_) {
+        project_cones(xp_, false, nullptr);
+        hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, dirn_, md, n_, 1, red(),
+                           ds_, inner_tol_, inner_limit_);
+    }
     const int *gate = &ds_->bb_active;
     const int max_bodies = inner_limit_ - 1;
     int issued = 0, chunk = 2;
@@ -563,8 +770,15 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
             q_product(xp_, qx_, gate);
             hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_, grad_,
                                dirn_, inv, n_, precond_ ? 1 : 0, red(), ds_);
+            if (has_cones_)
+                hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, snap_, xp_, n_, gate);
             hipLaunchKernelGGL(bb_direction_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, grad_, dirn_, lb_, ub_,
-                               mi, md, n_, red(), ds_, inner_tol_, inner_limit_);
+                               mi, md, n_, red(), ds_, inner_tol_, inner_limit_, defer);
+            if (has_cones_) {
+                project_cones(xp_, false, gate);
+                hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, snap_, dirn_, md, n_,
+                                   0, red(), ds_, inner_tol_, inner_limit_);
+            }
         }
         issued += todo;
         HIP_OK(hipGetLastError());
@@ -590,12 +804,20 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
     const int store = (is_major_ || ((tc + 2) % print_frequency(tc + 2)) == 0) ? 1 : 0;
     const double w = (double)(inner_count_ + 1) / (inner_count_ + 2);
     const double rho = P_.reflection_coefficient;
-    if (qtype_ == PDHCG_NON_Q || qtype_ == PDHCG_DIAG_Q) {
+    if (closed_prox()) {
+        // cones force the stored form and replace the reflection on cone slots
+        // after the projection (reference :428,730-735), so no Halpern fusion
+        const bool cones = has_cones_;
         EpiPrimalStep<X> epi{x_, xp_, xr_, dslack_, c_, lb_, ub_, x0_, qtype_ == PDHCG_DIAG_Q ? qdiag_ : nullptr,
-                             tau, w, rho, store, fuse_halpern ? 1 : 0};
+                             tau, w, rho, cones ? 1 : store, fuse_halpern ? 1 : 0};
         const int t = timer_.start(kTimedPrimalStep, st_);
         spmv_launch<V, X, Acc>(At_, pAt_, y_, epi, st_);
         timer_.stop(kTimedPrimalStep, t, st_);
+        if (cones) {
+            project_cones(xp_, false, nullptr);
+            hipLaunchKernelGGL(cone_reflect_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, xr_, xp_, x_, cones_.mask,
+                               n_);
+        }
     } else {
         const int t = timer_.start(kTimedPrimalStep, st_);
         spmv_launch<V, X, Acc>(At_, pAt_, y_, EpiStore<X>{aty_}, st_);
@@ -691,18 +913,52 @@ template <class V, class X> void Solver<V, X>::residual() {
     GridRed r1 = r0;
     r1.out = ds_->red + 3;
     hipLaunchKernelGGL(residual_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, aty_, qx_, xp_, lb_, ub_,
-                       dslack_, var_rescale_, tau, is_lp_ ? 0 : 1, n_, r1);
+                       dslack_, var_rescale_, tau, is_lp_ ? 0 : 1, n_, has_cones_ ? dres_ : nullptr, r1);
+    if (has_cones_) {
+        // red[12..13] augmented dual residual, [14..17] power violation, [18] cone slack correction
+        const X *qx = is_lp_ ? nullptr : qx_;
+        cones_.dual_residual(dres_, pgp_, c_, qx, aty_, var_rescale_, n_, st_);
+        // projected-gradient mapping (reference src/pdhg_core_op.cu:67-90)
+        double t = step_ / pw_;
+        if (!(t > 0.0) || !std::isfinite(t)) t = 1.0;
+        hipLaunchKernelGGL(projected_gradient_point_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, pgp_, xp_, c_, qx,
+                           aty_, lb_, ub_, t, n_);
+        project_cones(pgp_, true, nullptr);
+        GridRed r2 = r0;
+        r2.out = ds_->red + 12;
+        hipLaunchKernelGGL(augment_dual_residual_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, dres_, xp_, pgp_,
+                           var_rescale_, t, n_, r2);
+        if (has_power_) {
+            GridRed r3 = r0;
+            r3.out = ds_->red + 14;
+            hipLaunchKernelGGL(power_violation_kernel<X>, dim3(grid(cones_.powc.count)), dim3(kBlock), 0, st_, xp_,
+                               var_rescale_, cones_.powc, cbr_, r3);
+        }
+        GridRed r4 = r0;
+        r4.out = ds_->red + 18;
+        hipLaunchKernelGGL(cone_slack_correction_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, qx, aty_,
+                           dslack_, xp_, cones_.mask, n_, r4);
+    }
     HIP_OK(hipGetLastError());
     read_scalars();
     const double *R = hs_->red;
     // rows: [sum pres^2, row terms, max|pres|]; cols: [sum dres^2, c'x, x'Qx, slack'x, (Qx)^2, (A'y)^2, max dres, max Qx, max A'y]
     abs_pr_ = (linf ? R[2] : std::sqrt(R[0])) / cbr_;
     const double lin_pr = abs_pr_;
-    abs_du_ = (linf ? R[3 + 6] : std::sqrt(R[3 + 0])) / ovr_;
+    double pv_rel = 0.0;
+    if (has_power_) {  // reference src/pdhg_core_op.cu:214-270
+        const double pv_abs = (linf ? R[16] : std::sqrt(R[14])) / cbr_;
+        pv_rel = linf ? R[17] : std::sqrt(R[15]);
+        abs_pr_ = linf ? std::fmax(abs_pr_, pv_abs) : std::hypot(abs_pr_, pv_abs);
+    }
+    if (has_cones_) abs_du_ = (linf ? R[13] : std::sqrt(R[12])) / ovr_;
+    else abs_du_ = (linf ? R[3 + 6] : std::sqrt(R[3 + 0])) / ovr_;
     const double half_xqx = qtype_ == PDHCG_NON_Q ? 0.0 : 0.5 * R[3 + 2];
+    const double slack_x = R[3 + 3] + (has_cones_ ? R[18] : 0.0);
     pobj_ = (R[3 + 1] + half_xqx) / (cbr_ * ovr_) + obj_const_;
-    dobj_ = (R[3 + 3] + R[1] - half_xqx) / (cbr_ * ovr_) + obj_const_;
+    dobj_ = (slack_x + R[1] - half_xqx) / (cbr_ * ovr_) + obj_const_;
     rel_pr_ = lin_pr / (1.0 + bound_norm_);
+    if (has_power_) rel_pr_ = linf ? std::fmax(rel_pr_, pv_rel) : std::hypot(rel_pr_, pv_rel);
     double den;
     if (is_lp_) den = 1.0 + obj_norm_;
     else {
@@ -772,6 +1028,11 @@ template <class V, class X> void Solver<V, X>::check_termination() {
         reason_ = TERMINATION_REASON_OPTIMAL;
         return;
     }
+    if (has_cones_) {  // ray certificates are box-only (reference src/utils.cu:263-278)
+        if (total_count_ >= c.iteration_limit) reason_ = TERMINATION_REASON_ITERATION_LIMIT;
+        else if (cum_time_ >= c.time_sec_limit) reason_ = TERMINATION_REASON_TIME_LIMIT;
+        return;
+    }
     if (dray_obj_ > 0.0 && max_dray_ / dray_obj_ <= c.eps_infeasible) {
         reason_ = TERMINATION_REASON_PRIMAL_INFEASIBLE;
         return;
@@ -800,7 +1061,7 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
         const int tc = total_count_;
         if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
             residual();
-            if (is_major_ && tc < 3 * ef) infeasibility();
+            if (!has_cones_ && is_major_ && tc < 3 * ef) infeasibility();
             cum_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start_).count();
             check_termination();
             if (P_.verbose >= 2 && tc % print_frequency(tc) == 0)
@@ -816,15 +1077,18 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
         is_major_ = ((tc + 1) % ef) == 0;
         const bool need_fpe = is_major_ || do_restart_;
         const double w = (double)(inner_count_ + 1) / (inner_count_ + 2);
-        pdhg_step(!need_fpe);
+        // the closed-form prox with cones rewrites x_bar after the step, so
+        // its Halpern update cannot ride in the SpMV epilogue
+        const bool fuse = !need_fpe && !(has_cones_ && closed_prox());
+        pdhg_step(fuse);
         if (need_fpe) {
             fixed_point_error();
             if (do_restart_) {
                 init_fpe_ = fpe_;
                 do_restart_ = false;
             }
-            halpern(w);
         }
+        if (!fuse) halpern(w);
         inner_count_++;
         total_count_++;
     }
@@ -937,10 +1201,16 @@ pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_paramet
         std::fprintf(stderr, "[interface] pdhcg_solver_create: invalid arguments.\n");
         return nullptr;
     }
-    if (prob->cones.num_cones > 0 || prob->affine_cones.num_cones > 0) {
-        std::fprintf(stderr, "[interface] cone blocks are not yet supported by this build's device solver.\n");
+    if (prob->affine_cones.num_cones > 0) {
+        std::fprintf(stderr, "[interface] affine cone blocks are not supported by this build's device solver.\n");
         return nullptr;
     }
+    for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i)
+        if (prob->cones.is_fixed[i]) {
+            std::fprintf(stderr, "[interface] fixed cone sections are not supported by this build's device solver.\n");
+            return nullptr;
+        }
+
     if (prob->num_quadratic_constraints > 0) {
         std::fprintf(stderr, "Error: QCQP -> SOCP transformation is not available; cannot solve.\n");
         return nullptr;
@@ -1033,6 +1303,54 @@ void pdhcg_solver_free(pdhcg_solver_t *s) {
         delete s;
     } catch (...) {
     }
+}
+
+// Standalone cone operations on host vectors (one device round trip each).
+namespace {
+int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, const double *in, double *out, int n,
+               bool dual) {
+    if (!cones || !in || !out || n < 0 || cones->num_cones < 0 || (cones->num_cones > 0 && !cones->start_idx)) {
+        std::fprintf(stderr, "[interface] %s: invalid arguments.\n", name);
+        return -1;
+    }
+    try {
+        hipStream_t st;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        DevConeSet set;
+        set.build(cones, n);
+        std::vector<double> dh((size_t)n, 1.0);
+        if (d) std::copy(d, d + n, dh.begin());
+        std::vector<double> vh(in, in + n);
+        double *dd = dupload<double>(dh, n), *v = dupload<double>(vh, n), *res = nullptr, *scratch = nullptr,
+               *zero = dzeros<double>(n);
+        if (dual) {
+            res = dupload<double>(std::vector<double>(out, out + n), n);
+            scratch = dzeros<double>(n);
+            set.dual_residual<double>(res, scratch, v, nullptr, zero, dd, n, st);
+        } else {
+            set.project<double>(v, dd, false, nullptr, st);
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out, dual ? res : v, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (void *p : {(void *)dd, (void *)v, (void *)res, (void *)scratch, (void *)zero})
+            if (p) (void)hipFree(p);
+        set.release();
+        (void)hipStreamDestroy(st);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s: %s\n", name, e.what());
+        return -1;
+    }
+}
+}  // namespace
+
+int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n) {
+    return cone_entry("pdhcg_project_cones", cones, d, x, x, n, false);
+}
+
+int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n) {
+    return cone_entry("pdhcg_cone_dual_residual", cones, d, rc, out, n, true);
 }
 
 pdhcg_result_t *solve_qp_problem(const qp_problem_t *prob, const pdhg_parameters_t *params) {
