@@ -55,13 +55,17 @@ struct orc_solver {
     double *eff;                  /* c + Qx, the effective objective gradient (QP with cones) */
     double *snap;                 /* BB primal snapshot */
     double *pgp;                  /* projected-gradient point */
+    /* affine cones on the rows of [A; F] (reference internal/internal_types.h:106-127,
+       src/solver_state.cu:621-638): projections onto diag(1 / con_rescale) K */
+    const cone_blocks_t *acb;
+    int has_affine;
+    double *aff_d, *aff_pw, *aff_rw; /* row scaling of the cones, warm starts */
+    double *aproj, *amem, *acomp;    /* dual prox point, dual membership, complementarity */
 };
 
-/* Applies every variable cone block's projection to v (reference
+/* Applies every block's projection onto diag(d) K to v (reference
  * src/cone_dispatch.cu:38-57). */
-static void project_cones(orc_solver_t *s, double *v, double *warm) {
-    const cone_blocks_t *b = s->cb;
-    const double *d = s->w->var_rescale;
+static void project_blocks(const cone_blocks_t *b, const double *d, double *v, double *warm) {
     for (int k = 0; k < b->num_cones; ++k) {
         int st = b->start_idx[k], vd = b->v_dim[k];
         switch (b->type[k]) {
@@ -73,6 +77,11 @@ static void project_cones(orc_solver_t *s, double *v, double *warm) {
         default: break;
         }
     }
+}
+
+/* Variable cone blocks: projection onto diag(var_rescale) K. */
+static void project_cones(orc_solver_t *s, double *v, double *warm) {
+    project_blocks(s->cb, s->w->var_rescale, v, warm);
 }
 
 /* x_bar = 2 x_p - x on every cone slot (reference src/cone_dispatch.cu:118-135). */
@@ -338,8 +347,8 @@ orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_
     if (!prob) return NULL;
     int any_fixed = 0;
     for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i) any_fixed |= prob->cones.is_fixed[i] != 0;
-    if (prob->affine_cones.num_cones > 0 || prob->num_quadratic_constraints > 0 || any_fixed) {
-        fprintf(stderr, "[oracle] affine cones, fixed cone sections and quadratic constraints are not restated.\n");
+    if (prob->num_quadratic_constraints > 0 || any_fixed) {
+        fprintf(stderr, "[oracle] fixed cone sections and quadratic constraints are not restated.\n");
         return NULL;
     }
     orc_solver_t *s = (orc_solver_t *)calloc(1, sizeof(orc_solver_t));
@@ -416,6 +425,17 @@ orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_
         fprintf(stderr, "[oracle] the diagonal-Q cone proximal step is not restated.\n");
         orc_solver_free(s);
         return NULL;
+    }
+    s->acb = &work->affine_cones;
+    s->has_affine = work->affine_cones.num_cones > 0 && m > 0;
+    if (s->has_affine) {
+        s->aff_d = zeros(m);
+        for (int i = 0; i < m; ++i) s->aff_d[i] = 1.0 / w->con_rescale[i];
+        s->aff_pw = zeros(work->affine_cones.num_cones);
+        s->aff_rw = zeros(work->affine_cones.num_cones);
+        s->aproj = zeros(m);
+        s->amem = zeros(m);
+        s->acomp = zeros(work->affine_cones.num_cones);
     }
     s->proj_warm = zeros(work->cones.num_cones);
     s->res_warm = zeros(work->cones.num_cones);
@@ -585,10 +605,14 @@ static void pdhg_step(orc_solver_t *s) {
     s->inner_total++;
     spmv(&w->A, s->xr, s->ax);
     if (m == 0) return;
+    if (s->has_affine) { /* reference src/pdhg_core_op.cu:809-829, kernels :252-281 */
+        for (int i = 0; i < m; ++i) s->aproj[i] = clampd(s->ax[i] + w->g[i] - s->y[i] / sigma, w->cl[i], w->cu[i]);
+        project_blocks(s->acb, s->aff_d, s->aproj, s->aff_pw);
+    }
     for (int i = 0; i < m; ++i) {
         double cur = s->y[i];
         double v = s->ax[i] + w->g[i];
-        double pj = clampd(v - cur / sigma, w->cl[i], w->cu[i]);
+        double pj = s->has_affine ? s->aproj[i] : clampd(v - cur / sigma, w->cl[i], w->cu[i]);
         double nx = cur - sigma * v + sigma * pj;
         if (store) s->yp[i] = nx;
         s->yr[i] = 2.0 * nx - cur;
@@ -732,6 +756,34 @@ static void residual(orc_solver_t *s) {
             if (fabs(r) > fabs(s->dres[j])) s->dres[j] = r;
         }
     }
+    double amem_norm = 0.0, acomp_norm = 0.0;
+    if (s->has_affine) {
+        /* primal residual against the affine cones and dual-cone membership of y
+           (reference src/pdhg_core_op.cu:1125-1129,114-201, kernels
+           pdhcg_affine_cone_kernels.cu:23-75, pdhcg_kernels.cu:660-671) */
+        const cone_blocks_t *b = s->acb;
+        for (int i = 0; i < m; ++i) {
+            s->pres[i] = clampd(s->ax[i] + w->g[i], w->cl[i], w->cu[i]);
+            s->amem[i] = 0.0;
+        }
+        project_blocks(b, s->aff_d, s->pres, s->aff_rw);
+        for (int k = 0; k < b->num_cones; ++k) {
+            int st = b->start_idx[k], L = orc_cone_block_length(b, k);
+            double dotv = 0.0;
+            for (int i = st; i < st + L; ++i) {
+                s->amem[i] = -s->yp[i];
+                dotv += s->yp[i] * (s->ax[i] + w->g[i]);
+            }
+            s->acomp[k] = fabs(dotv) / w->cbr;
+        }
+        project_blocks(b, s->aff_d, s->amem, s->aff_rw);
+        for (int i = 0; i < m; ++i) {
+            s->pres[i] = (s->ax[i] + w->g[i] - s->pres[i]) * w->con_rescale[i];
+            s->amem[i] *= s->aff_d[i];
+        }
+        amem_norm = linf ? inf_norm(m, s->amem) : nrm2(m, s->amem);
+        acomp_norm = linf ? inf_norm(b->num_cones, s->acomp) : nrm2(b->num_cones, s->acomp);
+    }
     s->abs_pr = (linf ? inf_norm(m, s->pres) : nrm2(m, s->pres)) / w->cbr;
     double lin_pr = s->abs_pr;
     double pv_abs = 0.0, pv_rel = 0.0;
@@ -749,7 +801,11 @@ static void residual(orc_solver_t *s) {
         pv_abs /= w->cbr;
         s->abs_pr = linf ? fmax(s->abs_pr, pv_abs) : hypot(s->abs_pr, pv_abs);
     }
-    s->abs_du = (linf ? inf_norm(n, s->dres) : nrm2(n, s->dres)) / w->ovr;
+    s->abs_du = linf ? inf_norm(n, s->dres) : nrm2(n, s->dres);
+    if (s->has_affine) /* reference src/pdhg_core_op.cu:1166-1189 */
+        s->abs_du = linf ? fmax(fmax(s->abs_du, amem_norm), acomp_norm)
+                         : sqrt(s->abs_du * s->abs_du + amem_norm * amem_norm + acomp_norm * acomp_norm);
+    s->abs_du /= w->ovr;
     double half_xqx = w->qtype == PDHCG_NON_Q ? 0.0 : 0.5 * dot(n, s->xp, s->qx);
     s->pobj = (dot(n, w->c, s->xp) + half_xqx) / (w->cbr * w->ovr) + w->obj_const;
     if (s->has_cones) { /* cone dual slack (reference src/cone_dispatch.cu:137-154) */
@@ -886,7 +942,7 @@ int orc_solver_run(orc_solver_t *s, int iters) {
         int tc = s->total_count;
         if (s->is_major || tc == 0 || tc % print_freq(tc) == 0) {
             residual(s);
-            if (!s->has_cones && s->is_major && tc < 3 * ef) infeasibility(s);
+            if (!s->has_cones && !s->has_affine && s->is_major && tc < 3 * ef) infeasibility(s);
             s->cum_time = elapsed(s);
             check_termination(s);
             if (s->reason != TERMINATION_REASON_UNSPECIFIED) break;
@@ -976,7 +1032,8 @@ void orc_solver_free(orc_solver_t *s) {
     double *arr[] = {s->x0, s->x, s->xp, s->xr, s->y0, s->y, s->yp, s->yr, s->aty, s->ax, s->dslack, s->pslack,
                      s->pres, s->dres, s->dx, s->dy, s->qx, s->rx, s->rxbuf, s->cl_fin, s->cu_fin, s->lb_fin,
                      s->ub_fin, s->grad, s->dirn, s->pbuf, s->diag_h, s->mdiag, s->minv,
-                     s->proj_warm, s->res_warm, s->eff, s->snap, s->pgp};
+                     s->proj_warm, s->res_warm, s->eff, s->snap, s->pgp, s->aff_d, s->aff_pw, s->aff_rw,
+                     s->aproj, s->amem, s->acomp};
     for (size_t i = 0; i < sizeof(arr) / sizeof(arr[0]); ++i) free(arr[i]);
     free(s);
 }

@@ -267,13 +267,19 @@ def carray(a) -> np.ndarray:
 
 
 def cone_specs(specs) -> Any:
-    """Builds a cone_spec_t array from (type, start, v_dim, alpha) tuples."""
+    """Builds a cone_spec_t array from (type, start, v_dim[, alpha[, is_fixed]])
+    tuples; is_fixed is a per-slot 0/1 sequence (or None)."""
     arr = (cone_spec_t * max(1, len(specs)))()
+    arr.keep = []
     for i, s in enumerate(specs):
         t, start, vd = s[0], s[1], s[2]
         arr[i].type, arr[i].start_idx, arr[i].v_dim = int(t), int(start), int(vd)
         arr[i].power_alpha = float(s[3]) if len(s) > 3 else 0.0
         arr[i].is_fixed = None
+        if len(s) > 4 and s[4] is not None:
+            mask = (C.c_char * len(s[4]))(*[bytes([1 if v else 0]) for v in s[4]])
+            arr.keep.append(mask)
+            arr[i].is_fixed = C.cast(mask, C.c_void_p)
     return arr
 
 

@@ -573,6 +573,107 @@ __global__ void __launch_bounds__(kBlock) cone_reflect_kernel(X *xr, const X *xp
         if (mask[i]) xr[i] = (X)(2.0 * (double)xp[i] - (double)x[i]);
 }
 
+// ---------------- affine cones on constraint rows ----------------
+// Rows of [A; F] whose value F x + g must lie in a cone.  The cones are
+// applied in the scaled space as diag(d) K with d = 1 / row scaling
+// (reference src/solver_state.cu:621-638).
+
+// Dual step, first half, fused into the A x_bar SpMV: v = A x_bar + g is
+// stored, and p = clamp(v - y / sigma, l, u) is written for the projection
+// (reference pdhcg_kernels.cu:252-264).
+template <class X> struct EpiDualPrepare {
+    X *v_out, *p_out;
+    const X *y, *g, *lo, *hi;
+    double sigma;
+    template <class Acc> __device__ void operator()(int r, Acc s) const {
+        const double v = (double)s + (double)g[r];
+        v_out[r] = (X)v;
+        p_out[r] = (X)fmax((double)lo[r], fmin(v - (double)y[r] / sigma, (double)hi[r]));
+    }
+};
+
+// Dual step, second half after the cone projection of p:
+// y+ = y - sigma v + sigma p, reflection (reference pdhcg_kernels.cu:266-281).
+template <class X>
+__global__ void __launch_bounds__(kBlock) affine_dual_finish_kernel(const X *y, X *yp, X *yr, const X *v, const X *p,
+                                                                     double sigma, int store, int m) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const double cur = (double)y[i];
+        const double nx = cur - sigma * (double)v[i] + sigma * (double)p[i];
+        if (store) yp[i] = (X)nx;
+        yr[i] = (X)(2.0 * nx - cur);
+    }
+}
+
+// KKT rows with affine cones, first half: pres = clamp(A x + g) (projected
+// onto the cones next), the dual-membership point starts at zero, and the
+// dual-objective row terms are reduced: out = [sum row terms]
+// (reference pdhcg_kernels.cu:660-676).
+template <class X>
+__global__ void __launch_bounds__(kBlock) affine_rows_prepare_kernel(const X *ax, const X *g, const X *cl, const X *cu,
+                                                                      const X *yp, const X *cl_fin, const X *cu_fin,
+                                                                      X *pres, X *amem, int m, GridRed gr) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const double val = (double)ax[i] + (double)g[i];
+        pres[i] = (X)fmax((double)cl[i], fmin(val, (double)cu[i]));
+        amem[i] = (X)0.0;
+        const double y = (double)yp[i];
+        v[0] += fmax(y, 0.0) * (double)cl_fin[i] + fmin(y, 0.0) * (double)cu_fin[i] - (double)g[i] * y;
+    }
+    grid_reduce<1, 0>(gr, v);
+}
+
+// Complementarity |<y, F x + g>| / cbr per affine block, and the dual
+// membership point -y on its rows; one wave per block, fixed-order sum
+// (reference pdhcg_affine_cone_kernels.cu:40-75).
+template <class X>
+__global__ void __launch_bounds__(kBlock) affine_complementarity_kernel(const X *ax, const X *g, const X *yp, X *amem,
+                                                                         const int *start, const int *len, int count,
+                                                                         double cbr, double *comp) {
+    const int b = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (b >= count) return;
+    const int s = start[b], L = len[b];
+    double dot = 0.0;
+    for (int j = lane; j < L; j += kWave) {
+        const double y = (double)yp[s + j];
+        amem[s + j] = (X)(-y);
+        dot += y * ((double)ax[s + j] + (double)g[s + j]);
+    }
+    dot = wave_sum(dot);
+    if (lane == 0) comp[b] = fabs(dot) / cbr;
+}
+
+// KKT rows with affine cones, second half: pres = (v - P(clamp v)) r and the
+// dual membership scaled by d; out = [sum pres^2, sum amem^2, max |pres|,
+// max |amem|] (reference pdhcg_affine_cone_kernels.cu:23-38).
+template <class X>
+__global__ void __launch_bounds__(kBlock) affine_rows_finish_kernel(const X *ax, const X *g, const X *pres,
+                                                                     const X *amem, const double *con_rescale,
+                                                                     const double *d, int m, GridRed gr) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const double pr = ((double)ax[i] + (double)g[i] - (double)pres[i]) * con_rescale[i];
+        const double am = (double)amem[i] * d[i];
+        v[0] += pr * pr;
+        v[1] += am * am;
+        v[2] = fmax(v[2], abs_or_inf(pr));
+        v[3] = fmax(v[3], abs_or_inf(am));
+    }
+    grid_reduce<2, 2>(gr, v);
+}
+
+// out = [sum a^2, max |a|] of a double vector.
+__global__ void __launch_bounds__(kBlock) sq_max_kernel(const double *a, int n, GridRed gr) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += a[i] * a[i];
+        v[1] = fmax(v[1], abs_or_inf(a[i]));
+    }
+    grid_reduce<1, 1>(gr, v);
+}
+
 // PSD dual residual, first half: rc = c_eff - A'y on PSD slots, kept in dres
 // and copied into p for the in-place projection.
 template <class X>

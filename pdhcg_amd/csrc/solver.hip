@@ -102,6 +102,8 @@ class DevConeSet {
     ConeList rsoc, soc, expc, powc;
     std::vector<PsdGroup> groups;
     char *mask = nullptr;
+    int total = 0;                               // all blocks, in input order:
+    int *all_start = nullptr, *all_len = nullptr;  // first slot and slot count
 
     DevConeSet() = default;
     DevConeSet(const DevConeSet &) = delete;
@@ -115,8 +117,11 @@ class DevConeSet {
         } lists[4];  // rotated SOC, SOC, exponential, power
         std::vector<char> hmask((size_t)n, 0);
         std::vector<std::pair<int, int>> ps;  // (order, start)
+        std::vector<int> hs, hl;
         for (int k = 0; k < b->num_cones; ++k) {
             const int st = b->start_idx[k], vd = b->v_dim[k], len = cone_block_slots(b, k);
+            hs.push_back(st);
+            hl.push_back(len);
             const cone_type_t t = b->type[k];
             if (st < 0 || len < 0 || (long long)st + len > n) fail("cone block %d lies outside the variables", k);
             if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone)
@@ -160,6 +165,9 @@ class DevConeSet {
             i = j;
         }
         mask = dupload<char>(hmask, hmask.size());
+        total = (int)hs.size();
+        all_start = dupload<int>(hs, hs.size());
+        all_len = dupload<int>(hl, hl.size());
     }
 
     // Projects v onto the scaled cones D K block by block (reference
@@ -226,8 +234,12 @@ class DevConeSet {
             if (g.work) (void)hipFree(g.work);
         }
         groups.clear();
+        for (int *p : {all_start, all_len})
+            if (p) (void)hipFree(p);
         if (mask) (void)hipFree(mask);
         mask = nullptr;
+        all_start = all_len = nullptr;
+        total = 0;
     }
 
   private:
@@ -327,6 +339,11 @@ template <class V, class X> class Solver final : public SolverBase {
     // variable cone blocks, one list per type (reference internal/internal_types.h:106-127)
     bool has_cones_ = false, has_psd_ = false, has_power_ = false;
     DevConeSet cones_;
+    // affine cones on the rows of [A; F] (reference src/solver_state.cu:621-638)
+    bool has_affine_ = false;
+    DevConeSet acones_;
+    double *aff_d_ = nullptr, *acomp_ = nullptr;  // 1 / row scaling; per-block complementarity
+    X *aproj_ = nullptr, *apres_ = nullptr, *amem_ = nullptr;
     X *snap_ = nullptr, *pgp_ = nullptr;
     double *dres_ = nullptr;
 
@@ -489,6 +506,17 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     if (!sp.dual_start.empty())
         for (int i = 0; i < m_; ++i) ys[i] = sp.dual_start[i] * sp.con_rescale[i] * sp.ovr;
     setup_cones(sp, xs);
+    if (sp.affine_cones && sp.affine_cones->num_cones > 0 && m_ > 0) {
+        has_affine_ = true;
+        acones_.build(sp.affine_cones, m_);
+        std::vector<double> inv((size_t)m_);
+        for (int i = 0; i < m_; ++i) inv[i] = 1.0 / sp.con_rescale[i];
+        aff_d_ = dupload<double>(inv, inv.size());
+        acomp_ = dzeros<double>((size_t)acones_.total);
+        aproj_ = dzeros<X>(m_);
+        apres_ = dzeros<X>(m_);
+        amem_ = dzeros<X>(m_);
+    }
     x0_ = dupload<X>(xs, n_);
     if (has_cones_) project_cones(x0_, false, nullptr);
     hipLaunchKernelGGL(clamp_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, x0_, lb_, ub_, n_);
@@ -601,10 +629,11 @@ template <class V, class X> Solver<V, X>::~Solver() {
                     cu_fin_, lb_fin_, ub_fin_, con_rescale_, var_rescale_, x0_,     x_,           xp_,
                     xr_,     y0_,    y_,      yp_,     yr_,          aty_,        ax_,          dslack_,
                     dx_,     dy_,    qx_,     grad_,   dirn_,        mdiag_,      minv_,        diag_h_,
-                    red_partials_, red_counter_, ds_, snap_, pgp_, dres_};
+                    red_partials_, red_counter_, ds_, snap_, pgp_, dres_, aff_d_, acomp_, aproj_, apres_, amem_};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     cones_.release();
+    acones_.release();
     if (hs_) (void)hipHostFree(hs_);
     if (st_) (void)hipStreamDestroy(st_);
 }
@@ -825,6 +854,18 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
         bb_prox(tau, fuse_halpern, w);
     }
     host_inner_ += 1;
+    if (has_affine_) {
+        // the dual prox point is projected onto the affine cones between the
+        // SpMV and the update (reference src/pdhg_core_op.cu:809-829)
+        EpiDualPrepare<X> prep{ax_, aproj_, y_, g_, cl_, cu_, sigma};
+        const int t = timer_.start(kTimedDualStep, st_);
+        spmv_launch<V, X, Acc>(A_, pA_, xr_, prep, st_);
+        timer_.stop(kTimedDualStep, t, st_);
+        acones_.project(aproj_, aff_d_, false, nullptr, st_);
+        hipLaunchKernelGGL(affine_dual_finish_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, y_, yp_, yr_, ax_,
+                           aproj_, sigma, store, m_);
+        return;
+    }
     EpiDualStep<X> depi{y_, yp_, yr_, g_, cl_, cu_, y0_, sigma, w, rho, store, fuse_halpern ? 1 : 0};
     const int t = timer_.start(kTimedDualStep, st_);
     spmv_launch<V, X, Acc>(A_, pA_, xr_, depi, st_);
@@ -908,8 +949,28 @@ template <class V, class X> void Solver<V, X>::residual() {
     spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
     q_product(xp_, qx_, nullptr);
     GridRed r0 = red();
-    hipLaunchKernelGGL(residual_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_, yp_,
-                       con_rescale_, cl_fin_, cu_fin_, m_, r0);
+    if (has_affine_) {
+        // red[1] row terms, [20..23] rows (sum pres^2, sum amem^2, max pres, max amem), [24..25] complementarity
+        GridRed ra = r0;
+        ra.out = ds_->red + 1;
+        hipLaunchKernelGGL(affine_rows_prepare_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_,
+                           yp_, cl_fin_, cu_fin_, apres_, amem_, m_, ra);
+        acones_.project(apres_, aff_d_, true, nullptr, st_);
+        const int waves = kBlock / kWave;
+        hipLaunchKernelGGL(affine_complementarity_kernel<X>, dim3((acones_.total + waves - 1) / waves), dim3(kBlock), 0,
+                           st_, ax_, g_, yp_, amem_, acones_.all_start, acones_.all_len, acones_.total, cbr_, acomp_);
+        acones_.project(amem_, aff_d_, true, nullptr, st_);
+        GridRed rb = r0;
+        rb.out = ds_->red + 20;
+        hipLaunchKernelGGL(affine_rows_finish_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, apres_, amem_,
+                           con_rescale_, aff_d_, m_, rb);
+        GridRed rc = r0;
+        rc.out = ds_->red + 24;
+        hipLaunchKernelGGL(sq_max_kernel, dim3(grid(acones_.total)), dim3(kBlock), 0, st_, acomp_, acones_.total, rc);
+    } else {
+        hipLaunchKernelGGL(residual_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_, yp_,
+                           con_rescale_, cl_fin_, cu_fin_, m_, r0);
+    }
     GridRed r1 = r0;
     r1.out = ds_->red + 3;
     hipLaunchKernelGGL(residual_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, aty_, qx_, xp_, lb_, ub_,
@@ -943,7 +1004,8 @@ template <class V, class X> void Solver<V, X>::residual() {
     read_scalars();
     const double *R = hs_->red;
     // rows: [sum pres^2, row terms, max|pres|]; cols: [sum dres^2, c'x, x'Qx, slack'x, (Qx)^2, (A'y)^2, max dres, max Qx, max A'y]
-    abs_pr_ = (linf ? R[2] : std::sqrt(R[0])) / cbr_;
+    const double pres2 = has_affine_ ? R[20] : R[0], pres_max = has_affine_ ? R[22] : R[2];
+    abs_pr_ = (linf ? pres_max : std::sqrt(pres2)) / cbr_;
     const double lin_pr = abs_pr_;
     double pv_rel = 0.0;
     if (has_power_) {  // reference src/pdhg_core_op.cu:214-270
@@ -951,8 +1013,11 @@ template <class V, class X> void Solver<V, X>::residual() {
         pv_rel = linf ? R[17] : std::sqrt(R[15]);
         abs_pr_ = linf ? std::fmax(abs_pr_, pv_abs) : std::hypot(abs_pr_, pv_abs);
     }
-    if (has_cones_) abs_du_ = (linf ? R[13] : std::sqrt(R[12])) / ovr_;
-    else abs_du_ = (linf ? R[3 + 6] : std::sqrt(R[3 + 0])) / ovr_;
+    const double dres2 = has_cones_ ? R[12] : R[3 + 0], dres_max = has_cones_ ? R[13] : R[3 + 6];
+    if (has_affine_)  // reference src/pdhg_core_op.cu:1166-1189
+        abs_du_ = linf ? std::fmax(std::fmax(dres_max, R[23]), R[25]) : std::sqrt(dres2 + R[21] + R[24]);
+    else abs_du_ = linf ? dres_max : std::sqrt(dres2);
+    abs_du_ /= ovr_;
     const double half_xqx = qtype_ == PDHCG_NON_Q ? 0.0 : 0.5 * R[3 + 2];
     const double slack_x = R[3 + 3] + (has_cones_ ? R[18] : 0.0);
     pobj_ = (R[3 + 1] + half_xqx) / (cbr_ * ovr_) + obj_const_;
@@ -1061,7 +1126,7 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
         const int tc = total_count_;
         if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
             residual();
-            if (!has_cones_ && is_major_ && tc < 3 * ef) infeasibility();
+            if (!has_cones_ && !has_affine_ && is_major_ && tc < 3 * ef) infeasibility();
             cum_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start_).count();
             check_termination();
             if (P_.verbose >= 2 && tc % print_frequency(tc) == 0)
@@ -1079,7 +1144,7 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
         const double w = (double)(inner_count_ + 1) / (inner_count_ + 2);
         // the closed-form prox with cones rewrites x_bar after the step, so
         // its Halpern update cannot ride in the SpMV epilogue
-        const bool fuse = !need_fpe && !(has_cones_ && closed_prox());
+        const bool fuse = !need_fpe && !(has_cones_ && closed_prox()) && !has_affine_;
         pdhg_step(fuse);
         if (need_fpe) {
             fixed_point_error();
@@ -1199,10 +1264,6 @@ void pdhcg_request_cancel(int flag) { g_pdhcg_cancel = flag; }
 pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params, int precision) {
     if (!prob) {
         std::fprintf(stderr, "[interface] pdhcg_solver_create: invalid arguments.\n");
-        return nullptr;
-    }
-    if (prob->affine_cones.num_cones > 0) {
-        std::fprintf(stderr, "[interface] affine cone blocks are not supported by this build's device solver.\n");
         return nullptr;
     }
     for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i)

@@ -132,3 +132,26 @@ def random_problem(kind: str, seed: int = 0, n: int = 60, m: int = 30) -> QPData
         B = random_sparse(rng, max(2, n // 20), n, max(1, int(0.05 * (n // 20) * n)))
         return QPData(Q=(B.T @ B + sp.diags(rng.uniform(0.1, 1.0, n))).tocsr(), R=R, **base)
     raise ValueError(kind)
+
+
+def portfolio_tracking(n: int, k: int, seed: int = 1003, te_factor: float = 1.5) -> QPData:
+    """Long-only portfolio with a tracking-error limit as an affine SOC.
+
+    min 1/2 x'(diag(q) + R' D R) x + c'x  s.t.  1'x = 1,  0 <= x <= 0.1,
+    || D^1/2 R x || <= tau  (rows F = [D^1/2 R; 0; 0], g = (0, ..., 0, 0, tau)).
+    R is k x n with N(0, 1/k) entries, D = diag U[.01, .1], q ~ U[1e-4, 1e-2],
+    c = -(0.05 + 0.02 xi), tau = te_factor x the tracking error of equal weights.
+    """
+    rng = np.random.default_rng(seed)
+    R = rng.standard_normal((k, n)) * np.sqrt(1.0 / k)
+    D = rng.uniform(0.01, 0.1, k)
+    q = rng.uniform(1e-4, 1e-2, n)
+    c = -(0.05 + 0.02 * rng.standard_normal(n))
+    eq = np.full(n, 1.0 / n)
+    tau = te_factor * float(np.linalg.norm(np.sqrt(D) * (R @ eq)))
+    F = np.vstack([np.sqrt(D)[:, None] * R, np.zeros((2, n))])
+    g = np.zeros(k + 2)
+    g[-1] = tau
+    return QPData(c=c, A=sp.csr_matrix(np.ones((1, n))), con_lb=[1.0], con_ub=[1.0], var_lb=np.zeros(n),
+                  var_ub=np.full(n, 0.1), Q=sp.diags(q).tocsr(), R=R, D=D, F=sp.csr_matrix(F), g=g,
+                  affine_cones=[(1, 0, k)], fmt={"R": "dense"})

@@ -182,6 +182,7 @@ def _create(lib, prefix, p):
 
 @pytest.mark.parametrize("case", [
     "no_matrices", "bad_q_shape", "csr_bad_col", "overlap", "cone_box", "bad_alpha", "affine_cover", "psd_fixed_none",
+    "affine_fixed",
 ])
 def test_validation_rejects_like_oracle(case):
     n = 6
@@ -206,6 +207,10 @@ def test_validation_rejects_like_oracle(case):
     elif case == "affine_cover":
         kw["F"] = sp.eye(4, n, format="csr")
         kw["affine_cones"] = [(capi.CONE_STANDARD_SOC, 0, 1)]
+    elif case == "affine_fixed":
+        # reference test/test_affine_cones.c:247-268: fixed slots on an affine cone
+        kw["F"] = sp.eye(3, n, format="csr")
+        kw["affine_cones"] = [(capi.CONE_STANDARD_SOC, 0, 1, 0.0, [1, 0, 0])]
     elif case == "psd_fixed_none":
         kw["cones"] = [(capi.CONE_PSD, 0, 3)]  # valid: 6 slots, no fixed mask
     p = QPData(**kw)

@@ -287,3 +287,73 @@ def test_sparse_q_with_soc_e2e():
     assert s.status == "OPTIMAL"
     assert np.allclose(s.x, [3.0, 4.0, 3.0, 4.0, 5.0], atol=1e-4)
     assert abs(s.pobj + 1.5) < 1e-3
+
+
+# ---- affine cones: reference test/test_affine_cones.c:113-215 ----
+def affine_cases():
+    """(name, problem, norm, cone-preserving scaling, expected x) of the
+    reference's one-variable affine-cone tests."""
+    col = lambda vals: sp.csr_matrix(np.array(vals, dtype=float).reshape(3, 1))
+    return [
+        # min x^2/2 - 2x  s.t. (10x, 0, x + 9) in SOC  ->  x = 1
+        ("soc", QPData(c=[-2.0], Q=sp.csr_matrix([[1.0]]), F=col([10.0, 0.0, 1.0]), g=[0.0, 0.0, 9.0],
+                       affine_cones=[(capi.CONE_STANDARD_SOC, 0, 1)]), capi.NORM_TYPE_L_INF, False, 1.0),
+        # min -x  s.t. (x, 1, 1) in RSOC (2 s t >= x^2)  ->  x = sqrt 2
+        ("rsoc", QPData(c=[-1.0], F=col([1.0, 0.0, 0.0]), g=[0.0, 1.0, 1.0],
+                        affine_cones=[(capi.CONE_ROTATED_SOC, 0, 1)]), capi.NORM_TYPE_L2, True, np.sqrt(2.0)),
+        # min -x  s.t. (x, 1, 2) in K_exp  ->  x = ln 2
+        ("exp", QPData(c=[-1.0], F=col([1.0, 0.0, 0.0]), g=[0.0, 1.0, 2.0],
+                       affine_cones=[(capi.CONE_EXPONENTIAL, 0, 1)]), capi.NORM_TYPE_L_INF, True, np.log(2.0)),
+        # min -x  s.t. (1, 1, x) in K_0.3  ->  x = 1
+        ("power", QPData(c=[-1.0], F=col([0.0, 0.0, 1.0]), g=[1.0, 1.0, 0.0],
+                         affine_cones=[(capi.CONE_POWER, 0, 1, 0.3)]), capi.NORM_TYPE_L2, True, 1.0),
+    ]
+
+
+def affine_params(norm, preserving):
+    # reference test/test_affine_cones.c:72-86
+    return dict(optimality_norm=norm, use_cone_preserving_scaling=preserving, termination_evaluation_frequency=10,
+                eps=1e-7, iteration_limit=1000000, time_limit=30.0)
+
+
+def check_affine_result(s, expected):
+    # reference test/test_affine_cones.c:88-111
+    assert s is not None and s.status == "OPTIMAL", s and s.status
+    assert abs(s.x[0] - expected) <= 2e-4 * (1 + abs(expected)), (s.x[0], expected)
+    assert s.rel_primal <= 2e-6 and s.rel_dual <= 2e-6, (s.rel_primal, s.rel_dual)
+    assert abs(s.pobj - s.dobj) <= 2e-4 * (1 + abs(expected)), (s.pobj, s.dobj)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_affine_cone_known_answers(case):
+    name, p, norm, preserving, expected = affine_cases()[case]
+    check_affine_result(oracle_solve(p, **affine_params(norm, preserving)), expected)
+
+
+def test_scalar_bound_infeasible():
+    # reference test/test_affine_cones.c:217-245: 0 = -1 on an empty row
+    p = QPData(c=[0.0], A=sp.csr_matrix((1, 1)), con_lb=[-1.0], con_ub=[-1.0])
+    s = oracle_solve(p, **affine_params(capi.NORM_TYPE_L_INF, True))
+    assert s is not None and s.status == "PRIMAL_INFEASIBLE", s and s.status
+
+
+@pytest.mark.parametrize("n,k,seed", [(30, 5, 3), (60, 8, 5)])
+def test_affine_soc_portfolio_matches_slsqp(n, k, seed):
+    # independent pin of the affine-cone path: scipy's SLSQP on the same
+    # tracking-error portfolio (quadratic constraint written directly); the
+    # instances are chosen with the tracking limit active at the optimum
+    from scipy.optimize import minimize
+    from pdhcg_amd import workloads
+    p = workloads.portfolio_tracking(n, k, seed=seed, te_factor=1.0)
+    o = oracle_solve(p, eps=1e-8, iteration_limit=500000)
+    R, D, q, c, tau = p.R, p.D, p.Q.diagonal(), p.c, p.g[-1]
+    te2 = lambda x: (D * (R @ x)) @ (R @ x)
+    cons = [{"type": "eq", "fun": lambda x: x.sum() - 1, "jac": lambda x: np.ones(n)},
+            {"type": "ineq", "fun": lambda x: tau ** 2 - te2(x), "jac": lambda x: -2 * R.T @ (D * (R @ x))}]
+    r = minimize(lambda x: 0.5 * (q * x) @ x + 0.5 * te2(x) + c @ x, np.full(n, 1.0 / n),
+                 jac=lambda x: q * x + R.T @ (D * (R @ x)) + c, bounds=[(0, 0.1)] * n, constraints=cons,
+                 method="SLSQP", options={"ftol": 1e-14, "maxiter": 2000})
+    assert o.status == "OPTIMAL" and r.success
+    assert np.sqrt(te2(o.x)) >= 0.999 * tau           # the cone constraint is active
+    assert abs(o.pobj - r.fun) <= 1e-7 * (1 + abs(r.fun)), (o.pobj, r.fun)
+    assert np.max(np.abs(o.x - r.x)) <= 1e-5
