@@ -97,6 +97,36 @@ def run_gpu(p, precision, steps, warmup, eps, max_solve_s):
                 rel=(res.rel_primal, res.rel_dual, res.rel_gap), pobj=res.pobj)
 
 
+def copy_probe(mib):
+    """Launches 5 elementwise copies of a known size (read mib MiB, write mib
+    MiB) so that a rocprofv3 --pmc pass can calibrate FETCH_SIZE / WRITE_SIZE
+    against known bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "read_bytes": n * 4, "write_bytes": n * 4, "launches": 5}), flush=True)
+
+
+def measured_traffic(kernel_name, workload_key):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled by the copy-kernel calibration), when it was measured on this
+    workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(workload_key, {}).get(kernel_name)
+    return None if e is None else e.get("bytes_per_launch")
+
+
 def run_cpu_baseline(p, eps, budget_s):
     """FP64 CPU oracle (single thread) on a bounded sample: setup once, then
     as many main-loop iterations as fit in budget_s."""
@@ -131,7 +161,12 @@ def main():
     ap.add_argument("--max-solve-seconds", type=float, default=60.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the known-size copy used to calibrate rocprofv3 byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -191,7 +226,9 @@ def main():
                                                                         p.R.shape[0], p.c.size),
                    "precision": args.precision, "scale": args.scale},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(names[dom], "%s_scale%g" % (args.precision, args.scale)),
+                     "algorithmic_bytes": kb[dom]},
         "cpu_baseline": cpu,
         "time_to_tol_s": main_run["time_to_tol"],
         "tol": args.eps,

@@ -86,6 +86,22 @@ int orc_project_block(int type, double *x, int v_dim, const double *d, double al
 /* Dual-cone residual (rc - P_{K*}(rc)) scaled by d, written to out. */
 int orc_dual_residual_block(int type, const double *rc, int v_dim, const double *d, double alpha, double *out);
 
+/* Permutations and partition cuts (orc_permute.c; reference src/permute.cu,
+ * src/partition_utils.c, distributed/distributed_utils.cu:179-216).  method:
+ * permute_method_t values.  Return 1 on success, 0 on rejection. */
+qp_problem_t *orc_qp_problem_copy(const qp_problem_t *p);
+int orc_permute_problem(qp_problem_t *p, const int *row_perm, const int *col_perm);
+qp_problem_t *orc_permute_problem_copy(const qp_problem_t *p, const int *row_perm, const int *col_perm);
+int orc_validate_cone_permutation(const qp_problem_t *p, const int *col_perm);
+int orc_validate_affine_row_permutation(const qp_problem_t *p, const int *row_perm);
+void orc_generate_cone_aware_permutation(const qp_problem_t *p, int method, int block_size, int *perm);
+void orc_generate_affine_row_permutation(const qp_problem_t *p, int method, int block_size, int *perm);
+void orc_repermute_solution(pdhcg_result_t *r, const int *row_perm, const int *col_perm);
+void orc_balanced_cuts(const int *weights, int total, int parts, int *cuts);
+int orc_optimize_partition_cuts(int total, int parts, const int *forbidden_starts, const int *forbidden_ends,
+                                int num_forbidden, int *cuts);
+
 #ifdef __cplusplus
 }
 #endif
+
