@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = set()
-    for h in ("pdhcg.h", "pdhcg_amd.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h") and f != "pdhcg_types.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
@@ -35,7 +35,8 @@ def test_exports_every_declared_symbol():
     names = declared_symbols()
     assert {"create_qp_problem", "solve_qp_problem", "set_default_parameters", "qp_problem_free",
             "pdhcg_result_free", "set_start_values", "set_cone_fixed", "qcqp_to_socp_qp",
-            "solve_qp_problem_distributed", "pdhcg_solver_create", "pdhcg_solver_run"} <= names
+            "solve_qp_problem_distributed", "pdhcg_solver_create", "pdhcg_solver_run", "pdhcg_project_cones",
+            "permute_problem", "optimize_partition_cuts", "get_balanced_cuts"} <= names
     for n in names:
         assert hasattr(L, n), n
 
