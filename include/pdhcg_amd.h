@@ -48,7 +48,7 @@ void pdhcg_solver_free(pdhcg_solver_t *solver);
  * pdhcg_solver_timing(s, 1) clears and enables recording; afterwards
  * pdhcg_solver_kernel_time returns the summed duration and launch count of
  * one of the instrumented kernels:
- *   0  R x partial sums with the D product (dense low-rank R), ungated launches
+ *   0  R x per-chunk partial sums (dense low-rank R), ungated launches
  *   1  R'u plus the diagonal of Q (dense low-rank R), ungated launches
  *   2  A'y SpMV of the PDHG step (fused primal epilogue, or stored for BB)
  *   3  A x_bar SpMV of the PDHG step with the fused dual epilogue */

@@ -63,25 +63,54 @@ struct orc_solver {
     double *aproj, *amem, *acomp;    /* dual prox point, dual membership, complementarity */
 };
 
+/* Projection of one block (slots v[0..len)) onto diag(d) K. */
+static void project_one(cone_type_t type, double *v, int vd, const double *d, double *warm, double alpha) {
+    switch (type) {
+    case CONE_STANDARD_SOC: orc_soc_project(v, vd, d, warm); break;
+    case CONE_ROTATED_SOC: orc_rsoc_project(v, vd, d, warm); break;
+    case CONE_EXPONENTIAL: orc_exp_project(v, d); break;
+    case CONE_POWER: orc_power_project(v, d, alpha); break;
+    case CONE_PSD: orc_psd_project(v, vd); break;
+    default: break;
+    }
+}
+
 /* Applies every block's projection onto diag(d) K to v (reference
  * src/cone_dispatch.cu:38-57). */
 static void project_blocks(const cone_blocks_t *b, const double *d, double *v, double *warm) {
-    for (int k = 0; k < b->num_cones; ++k) {
-        int st = b->start_idx[k], vd = b->v_dim[k];
-        switch (b->type[k]) {
-        case CONE_STANDARD_SOC: orc_soc_project(v + st, vd, d + st, warm + k); break;
-        case CONE_ROTATED_SOC: orc_rsoc_project(v + st, vd, d + st, warm + k); break;
-        case CONE_EXPONENTIAL: orc_exp_project(v + st, d + st); break;
-        case CONE_POWER: orc_power_project(v + st, d + st, b->power_alpha[k]); break;
-        case CONE_PSD: orc_psd_project(v + st, vd); break;
-        default: break;
-        }
-    }
+    for (int k = 0; k < b->num_cones; ++k)
+        project_one(b->type[k], v + b->start_idx[k], b->v_dim[k], d + b->start_idx[k], warm + k,
+                    b->power_alpha ? b->power_alpha[k] : 0.0);
 }
 
 /* Variable cone blocks: projection onto diag(var_rescale) K. */
 static void project_cones(orc_solver_t *s, double *v, double *warm) {
     project_blocks(s->cb, s->w->var_rescale, v, warm);
+}
+
+/* Cone step of the diagonal-Q proximal update: the projection onto D K in
+ * the metric diag(w), w = 1 + tau q, computed as W^-1/2 P_{W^1/2 D K}(W^1/2 v)
+ * (reference src/cone_dispatch.cu:59-85 with the *_diag_q kernels, e.g.
+ * src/kernels/pdhcg_exp_cone_kernels.cu:624-700).  PSD blocks never get here
+ * (they take the BB path). */
+static void project_cones_diag_q(orc_solver_t *s, double *v, double tau) {
+    const cone_blocks_t *b = s->cb;
+    const double *d = s->w->var_rescale, *q = s->w->qdiag;
+    for (int k = 0; k < b->num_cones; ++k) {
+        int st = b->start_idx[k], L = orc_cone_block_length(b, k);
+        double *sw = (double *)malloc((size_t)L * sizeof(double)), *e = (double *)malloc((size_t)L * sizeof(double));
+        for (int i = 0; i < L; ++i) {
+            double wi = 1.0 + tau * q[st + i];
+            if (!(wi > 0.0)) wi = 1.0;
+            sw[i] = sqrt(wi);
+            v[st + i] *= sw[i];
+            e[i] = sw[i] * d[st + i];
+        }
+        project_one(b->type[k], v + st, b->v_dim[k], e, s->proj_warm + k, b->power_alpha ? b->power_alpha[k] : 0.0);
+        for (int i = 0; i < L; ++i) v[st + i] /= sw[i];
+        free(sw);
+        free(e);
+    }
 }
 
 /* x_bar = 2 x_p - x on every cone slot (reference src/cone_dispatch.cu:118-135). */
@@ -421,11 +450,6 @@ orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_
         s->has_power |= work->cones.type[k] == CONE_POWER;
         s->has_psd |= work->cones.type[k] == CONE_PSD;
     }
-    if (s->has_cones && w->qtype == PDHCG_DIAG_Q && !s->has_psd) {
-        fprintf(stderr, "[oracle] the diagonal-Q cone proximal step is not restated.\n");
-        orc_solver_free(s);
-        return NULL;
-    }
     s->acb = &work->affine_cones;
     s->has_affine = work->affine_cones.num_cones > 0 && m > 0;
     if (s->has_affine) {
@@ -595,7 +619,11 @@ static void pdhg_step(orc_solver_t *s) {
             double t = (xi - tau * (w->c[i] - s->aty[i])) / (1.0 + tau * w->qdiag[i]);
             double pj = clampd(t, w->lb[i], w->ub[i]);
             s->xr[i] = 2.0 * pj - xi;
-            if (store) s->xp[i] = pj;
+            if (store || s->has_cones) s->xp[i] = pj; /* cones force the stored form (reference :460-463) */
+        }
+        if (s->has_cones) { /* reference :742-746 */
+            project_cones_diag_q(s, s->xp, tau);
+            cone_reflection(s);
         }
         break;
     default:

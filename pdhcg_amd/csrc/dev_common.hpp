@@ -21,8 +21,9 @@ namespace pdhcg {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;            // threads per block of the elementwise/reduction kernels
-constexpr int kMaxRedBlocks = 2048;    // upper bound on blocks of any reducing launch
+constexpr int kMaxRedBlocks = 1024;    // upper bound on blocks of any reducing launch
 constexpr int kMaxRedValues = 16;      // values reduced by one launch
+constexpr size_t kRedPartials = (size_t)kMaxRedValues * kMaxRedBlocks;
 
 template <class T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -44,9 +45,11 @@ template <int NS> __device__ __forceinline__ double combine(int i, double a, dou
 }
 
 // Reduction request: the output receives NS sums followed by NM maxima (FP64).
+// A reducing kernel writes one partial pack per workgroup; the fold kernel
+// launched after it on the same stream combines them (no atomics, no fences:
+// the kernel boundary orders the writes before the reads).
 struct GridRed {
-    double *partials;   // [kMaxRedValues][kMaxRedBlocks]
-    unsigned *counter;  // arrival counter, zero between launches
+    double *partials;   // kRedPartials: [kMaxRedValues][kMaxRedBlocks]
     double *out;
 };
 
@@ -79,45 +82,53 @@ template <int NS, int NM> __device__ __forceinline__ void block_reduce(double (&
     __syncthreads();
 }
 
-// Grid reduction with a fixed combination order: every block stores its
-// partials, the last block to arrive folds them in block order, writes g.out
-// and returns true (in all of its threads); the other blocks return false.
-// The result does not depend on block scheduling.  Maxima start from -inf,
-// so callers reduce magnitudes (>= 0) or explicit values.
-template <int NS, int NM> __device__ bool grid_reduce(const GridRed &g, double (&v)[NS + NM]) {
+// Stores this workgroup's partial pack (thread 0 writes after a block
+// reduction in a fixed order).  Launch grid_fold_kernel with the same grid
+// size afterwards.
+template <int NS, int NM> __device__ void grid_partials(const GridRed &g, double (&v)[NS + NM]) {
     constexpr int NV = NS + NM;
     static_assert(NV <= kMaxRedValues, "too many reduced values");
     __shared__ double lds[NV * 16];
-    __shared__ int last;
     block_reduce<NS, NM>(v, lds);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int i = 0; i < NV; ++i) g.partials[(size_t)i * kMaxRedBlocks + blockIdx.x] = v[i];
-        __threadfence();
-        unsigned prev = atomicAdd(g.counter, 1u);
-        last = (prev == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// No follow-up work after a reduction.
+struct NoTail {
+    template <class S> __device__ void operator()(const double *, S *) const {}
+};
+
+// One workgroup folds the partial packs of `nblocks` workgroups in a fixed
+// order (thread t takes blocks t, t + 256, ... with four loads in flight),
+// writes g.out and hands the values to tail(values, scalars) in thread 0.
+// gate: skip everything when *gate == 0 (device-resident loops).
+template <int NS, int NM, class Tail, class S>
+__global__ void __launch_bounds__(kBlock) grid_fold_kernel(GridRed g, int nblocks, Tail tail, S *scalars,
+                                                            const int *gate) {
+    if (gate && *gate == 0) return;
+    constexpr int NV = NS + NM;
+    __shared__ double lds[NV * 16];
     double t[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) t[i] = i < NS ? 0.0 : -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+    for (int i = 0; i < NV; ++i) {
+        double a[4];
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-            t[i] = combine<NS>(i, t[i], coherent_load(g.partials + (size_t)i * kMaxRedBlocks + b));
+        for (int q = 0; q < 4; ++q) {
+            const int b = threadIdx.x + q * kBlock;
+            a[q] = b < nblocks ? g.partials[(size_t)i * kMaxRedBlocks + b] : (i < NS ? 0.0 : -INFINITY);
+        }
+        t[i] = combine<NS>(i, combine<NS>(i, a[0], a[1]), combine<NS>(i, a[2], a[3]));
+    }
     block_reduce<NS, NM>(t, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) g.out[i] = t[i];
-        *g.counter = 0u;
-#pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] = t[i];
+        tail(t, scalars);
     }
-    __syncthreads();
-    return true;
 }
+static_assert(kMaxRedBlocks <= 4 * kBlock, "the fold reads four partials per thread");
 
 inline int blocks_for(long long n, int per_block = kBlock, int cap = kMaxRedBlocks) {
     long long b = (n + per_block - 1) / per_block;

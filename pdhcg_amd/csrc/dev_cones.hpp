@@ -373,6 +373,43 @@ __global__ void __launch_bounds__(kBlock) cone_project_kernel(X *v, const double
     }
 }
 
+// Cone step of the diagonal-Q proximal update: the projection onto D K in
+// the metric diag(w), w = 1 + tau q, computed as W^-1/2 P_{W^1/2 D K}(W^1/2 v)
+// (reference src/cone_dispatch.cu:59-85 with the *_diag_q kernels, e.g.
+// pdhcg_exp_cone_kernels.cu:624-700).  Not used for PSD blocks.
+template <class X, int Type>
+__global__ void __launch_bounds__(kBlock) cone_project_diag_q_kernel(X *v, const double *d, const X *q, double tau,
+                                                                      ConeList L) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= L.count) return;
+    const int s = L.start[b];
+    const bool soc = Type == CONE_STANDARD_SOC || Type == CONE_ROTATED_SOC;
+    const int len = soc ? L.vdim[b] + 2 : 3;
+    auto root_w = [&](int m) {
+        const double w = 1.0 + tau * (double)q[s + m];
+        return sqrt(w > 0.0 ? w : 1.0);
+    };
+    double pv[kMaxThreadCone + 2], e[kMaxThreadCone + 2];
+    for (int m = 0; m < len; ++m) {
+        const double sw = root_w(m);
+        pv[m] = (double)v[s + m] * sw;
+        e[m] = sw * d[s + m];
+    }
+    if (soc) {
+        soc_project_block(pv, len - 2, e, Type == CONE_ROTATED_SOC, L.warm + b);
+    } else if (Type == CONE_EXPONENTIAL) {
+        double o[3];
+        exp_project_point(pv[0], pv[1], pv[2], e[0], e[1], e[2], o);
+        pv[0] = o[0]; pv[1] = o[1]; pv[2] = o[2];
+    } else {
+        double o[3];
+        power_project_point(pv[0] / e[0], pv[1] / e[1], pv[2] / e[2], e[0] * e[0], e[1] * e[1], e[2] * e[2],
+                            L.alpha[b], o);
+        pv[0] = o[0] * e[0]; pv[1] = o[1] * e[1]; pv[2] = o[2] * e[2];
+    }
+    for (int m = 0; m < len; ++m) v[s + m] = (X)(pv[m] / root_w(m));
+}
+
 // Dual-cone residual of rc = c_eff - A'y on one block, written into dres
 // (reference src/cone_dispatch.cu:88-115).  c_eff = c + qx when qx != null.
 template <class X, int Type>
@@ -442,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
         v[2] = fmax(v[2], av);
         v[3] = fmax(v[3], rv);
     }
-    grid_reduce<2, 2>(gr, v);
+    grid_partials<2, 2>(gr, v);
 }
 
 // ---------------- PSD cone: one workgroup per matrix ----------------
@@ -621,7 +658,7 @@ __global__ void __launch_bounds__(kBlock) affine_rows_prepare_kernel(const X *ax
         const double y = (double)yp[i];
         v[0] += fmax(y, 0.0) * (double)cl_fin[i] + fmin(y, 0.0) * (double)cu_fin[i] - (double)g[i] * y;
     }
-    grid_reduce<1, 0>(gr, v);
+    grid_partials<1, 0>(gr, v);
 }
 
 // Complementarity |<y, F x + g>| / cbr per affine block, and the dual
@@ -661,7 +698,7 @@ __global__ void __launch_bounds__(kBlock) affine_rows_finish_kernel(const X *ax,
         v[2] = fmax(v[2], abs_or_inf(pr));
         v[3] = fmax(v[3], abs_or_inf(am));
     }
-    grid_reduce<2, 2>(gr, v);
+    grid_partials<2, 2>(gr, v);
 }
 
 // out = [sum a^2, max |a|] of a double vector.
@@ -671,7 +708,7 @@ __global__ void __launch_bounds__(kBlock) sq_max_kernel(const double *a, int n, 
         v[0] += a[i] * a[i];
         v[1] = fmax(v[1], abs_or_inf(a[i]));
     }
-    grid_reduce<1, 1>(gr, v);
+    grid_partials<1, 1>(gr, v);
 }
 
 // PSD dual residual, first half: rc = c_eff - A'y on PSD slots, kept in dres

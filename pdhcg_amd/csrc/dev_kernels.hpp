@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) dot_kernel(const X *a, const X *b, int
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         v[0] += (double)a[i] * (double)b[i];
-    grid_reduce<1, 0>(g, v);
+    grid_partials<1, 0>(g, v);
 }
 
 // dx = a - b (n), dy = c - d (m); out = [|dx|^2, |dy|^2]
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) delta_norms_kernel(const X *a, const X
             v[1] += (double)(X)t * (double)(X)t;
         }
     }
-    grid_reduce<2, 0>(g, v);
+    grid_partials<2, 0>(g, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) delta_norms_kernel(const X *a, const X
 // M = diag(H) + 1/tau (non-positive -> 1), M^-1, tol_scale = sqrt(sum|M| / n).
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_refresh_kernel(const double *diag_h, double inv_tau, X *m_diag, X *m_inv,
-                                                             int n, GridRed g, DevScalars *ds) {
+                                                             int n, GridRed g) {
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double mm = diag_h[i] + inv_tau;
@@ -182,16 +182,22 @@ __global__ void __launch_bounds__(kBlock) bb_refresh_kernel(const double *diag_h
         m_inv[i] = (X)(1.0 / mm);
         v[0] += fabs(mm);
     }
-    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) ds->tol_scale = n > 0 ? sqrt(v[0] / (double)n) : 1.0;
+    grid_partials<1, 0>(g, v);
 }
+
+struct TailTolScale {  // tol_scale = sqrt(sum |M| / n)
+    int n;
+    __device__ void operator()(const double *r, DevScalars *ds) const {
+        ds->tol_scale = n > 0 ? sqrt(r[0] / (double)n) : 1.0;
+    }
+};
 
 // First projected gradient step from x_k (reference :525-559 with kernels :511-534,591-615).
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_init_kernel(const X *aty, X *grad, X *dirn, const X *x, X *xp,
                                                           const X *c, const X *qx, const X *lb, const X *ub,
                                                           const X *m_inv, const X *m_diag, double alpha0, int n,
-                                                          GridRed g, DevScalars *ds, double tol, int limit,
-                                                          int defer) {
+                                                          GridRed g, const DevScalars *ds, int defer) {
     const bool pre = m_inv != nullptr;
     const double a0 = pre ? alpha0 * (ds->tol_scale * ds->tol_scale) : alpha0;
     double v[1] = {0.0};
@@ -207,20 +213,29 @@ __global__ void __launch_bounds__(kBlock) bb_init_kernel(const X *aty, X *grad, 
         v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
     }
     if (defer) return;  // cones: the step is re-measured after the projection
-    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
-        ds->bb_iter = 1;
-        if (pre) ds->bb_stms = v[0];
-        ds->bb_nrm = sqrt(v[0]);
-        ds->bb_active = (1 < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
-    }
+    grid_partials<1, 0>(g, v);
 }
+
+// Stopping test after a step (reference :575-597): iteration count, |s| or
+// sqrt(s'Ms), and the active flag.  first: the step of bb_init (iteration 1).
+struct TailBBStep {
+    int first, precond, limit;
+    double tol;
+    __device__ void operator()(const double *r, DevScalars *ds) const {
+        const int it = first ? 1 : ds->bb_iter + 1;
+        ds->bb_iter = it;
+        if (precond) ds->bb_stms = r[0];
+        ds->bb_nrm = sqrt(r[0]);
+        ds->bb_active = (it < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
+    }
+};
 
 // g_new = Qx + c - A'y + (x_p - x)/tau; tmp = s'(g_new - g); alpha
 // (reference :599-650 with kernels :536-555,392-400).
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_gradient_kernel(const X *xp, const X *x, const X *c, const X *aty,
                                                               const X *qx, X *grad, const X *dirn, double inv_tau,
-                                                              int n, int precond, GridRed g, DevScalars *ds) {
+                                                              int n, GridRed g, const DevScalars *ds) {
     if (ds->bb_active == 0) return;
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -230,18 +245,23 @@ __global__ void __launch_bounds__(kBlock) bb_gradient_kernel(const X *xp, const 
         grad[i] = (X)cur;
         v[0] += (double)dirn[i] * dg;
     }
-    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
-        ds->bb_tmp = v[0];
-        ds->bb_alpha = precond ? ds->bb_stms / v[0] : (ds->bb_nrm * ds->bb_nrm) / v[0];
-    }
+    grid_partials<1, 0>(g, v);
 }
+
+struct TailBBAlpha {  // alpha = s'Ms / s'dg or |s|^2 / s'dg
+    int precond;
+    __device__ void operator()(const double *r, DevScalars *ds) const {
+        ds->bb_tmp = r[0];
+        ds->bb_alpha = precond ? ds->bb_stms / r[0] : (ds->bb_nrm * ds->bb_nrm) / r[0];
+    }
+};
 
 // x_p <- P(x_p - alpha [M^-1] g), s = step, next stopping test
 // (reference kernels :557-575,617-636 and loop head :575-597).
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *grad, X *dirn, const X *lb, const X *ub,
                                                                const X *m_inv, const X *m_diag, int n, GridRed g,
-                                                               DevScalars *ds, double tol, int limit, int defer) {
+                                                               const DevScalars *ds, int defer) {
     if (ds->bb_active == 0) return;
     const bool pre = m_inv != nullptr;
     const double a = ds->bb_alpha;
@@ -256,13 +276,7 @@ __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *gr
         v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
     }
     if (defer) return;
-    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
-        const int it = ds->bb_iter + 1;
-        ds->bb_iter = it;
-        if (pre) ds->bb_stms = v[0];
-        ds->bb_nrm = sqrt(v[0]);
-        ds->bb_active = (it < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
-    }
+    grid_partials<1, 0>(g, v);
 }
 
 // With cones: after the cone projection of x_p the step is s = x_p - base
@@ -270,8 +284,7 @@ __global__ void __launch_bounds__(kBlock) bb_direction_kernel(X *xp, const X *gr
 // stopping test uses that s (reference :561-566,652-657 and :575-597).
 template <class X>
 __global__ void __launch_bounds__(kBlock) bb_cone_step_kernel(const X *xp, const X *base, X *dirn, const X *m_diag,
-                                                               int n, int first, GridRed g, DevScalars *ds, double tol,
-                                                               int limit) {
+                                                               int n, int first, GridRed g, const DevScalars *ds) {
     if (!first && ds->bb_active == 0) return;
     const bool pre = m_diag != nullptr;
     double v[1] = {0.0};
@@ -280,13 +293,7 @@ __global__ void __launch_bounds__(kBlock) bb_cone_step_kernel(const X *xp, const
         dirn[i] = (X)s;
         v[0] += pre ? s * ((double)m_diag[i] * s) : s * s;
     }
-    if (grid_reduce<1, 0>(g, v) && threadIdx.x == 0) {
-        const int it = first ? 1 : ds->bb_iter + 1;
-        ds->bb_iter = it;
-        if (pre) ds->bb_stms = v[0];
-        ds->bb_nrm = sqrt(v[0]);
-        ds->bb_active = (it < limit && !(ds->bb_nrm <= tol)) ? 1 : 0;
-    }
+    grid_partials<1, 0>(g, v);
 }
 
 // Projected-gradient point for the conic stationarity residual
@@ -318,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) augment_dual_residual_kernel(double *d
         v[0] += dr * dr;
         v[1] = fmax(v[1], abs_or_inf(dr));
     }
-    grid_reduce<1, 1>(g, v);
+    grid_partials<1, 1>(g, v);
 }
 
 // Dual objective correction for cone slots: the cone dual slack is
@@ -335,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) cone_slack_correction_kernel(const X *
             v[0] += (cs - (double)dslack[i]) * (double)xp[i];
             dslack[i] = (X)cs;
         }
-    grid_reduce<1, 0>(g, v);
+    grid_partials<1, 0>(g, v);
 }
 
 // x_bar = 2 x_p - x_k, inner count, optional Halpern (reference :664-668, kernel :577-589).
@@ -358,69 +365,156 @@ __global__ void __launch_bounds__(kBlock) bb_final_kernel(X *x, const X *xp, X *
 // Low-rank product R'DRx with R dense row-major (k x n)
 // ---------------------------------------------------------------------------
 
-constexpr int kRxChunk = 2048;  // columns of R per workgroup (x chunk staged in LDS)
+constexpr int kRxLaneCols = 4;                      // columns of R per lane
+constexpr int kRxWaveCols = kWave * kRxLaneCols;    // columns of R per wave (one chunk)
 constexpr int kRxMaxRank = 256;
 
-// Per-workgroup partial sums of R x over a chunk of kRxChunk columns: the x
-// chunk is staged once in LDS, then each wave streams whole contiguous row
-// segments R[j, chunk] (coalesced, four loads in flight per lane) and reduces
-// them with one butterfly per (row, chunk).  The last workgroup folds the
-// partials in workgroup order, applies D and writes u = D R x (FP64).
+inline int rx_chunks(int n) { return (n + kRxWaveCols - 1) / kRxWaveCols; }
+
+// Sums four per-lane values s[0..3] over the wave with a transposed
+// butterfly: after the xor-32 and xor-16 exchanges each lane holds one row's
+// partial, so only 7 exchanges are needed instead of 24.  Returns the total of
+// row (lane / 16) in lanes 0, 16, 32 and 48.  Fixed combination order.
+__device__ __forceinline__ double wave_sum4(const double (&s)[4], int lane) {
+    const bool hi = lane >= 32, mid = (lane & 16) != 0;
+    const double k0 = hi ? s[2] : s[0], k1 = hi ? s[3] : s[1];
+    const double g0 = hi ? s[0] : s[2], g1 = hi ? s[1] : s[3];
+    const double t0 = k0 + __shfl_xor(g0, 32, kWave);
+    const double t1 = k1 + __shfl_xor(g1, 32, kWave);
+    double u = (mid ? t1 : t0) + __shfl_xor(mid ? t0 : t1, 16, kWave);
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) u += __shfl_xor(u, off, kWave);
+    return u;
+}
+
+// RU rows of R over this lane's kRxLaneCols columns: all RU * kRxLaneCols
+// loads are issued before the multiply-adds (one register block).
+template <int RU, class V>
+__device__ __forceinline__ void rx_rows(const V *R, size_t n, int j0, long long c0, const double (&xv)[kRxLaneCols],
+                                        double (&s)[RU]) {
+    double r[RU][kRxLaneCols];
+#pragma unroll
+    for (int a = 0; a < RU; ++a)
+#pragma unroll
+        for (int q = 0; q < kRxLaneCols; ++q) r[a][q] = (double)R[(size_t)(j0 + a) * n + c0 + q * kWave];
+#pragma unroll
+    for (int a = 0; a < RU; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < kRxLaneCols; ++q) t += r[a][q] * xv[q];
+        s[a] = t;
+    }
+}
+
+// Pass 1 of u = D R x: each wave owns kRxWaveCols consecutive columns (x for
+// them held in registers), streams the rows of R over that chunk eight rows
+// at a time (32 loads in flight per lane, coalesced) and writes one partial
+// sum per (row, chunk): partials[j * nchunks + chunk].  The last chunk, which
+// may extend past column n, takes a guarded path.
+template <class V, class X, bool Gated>
+__global__ void __launch_bounds__(kBlock) rx_partial_kernel(const V *R, const X *x, int n, int k, double *partials,
+                                                             int nchunks, const int *gate) {
+    if (Gated && *gate == 0) return;
+    const int lane = threadIdx.x % kWave;
+    const int chunk = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (chunk >= nchunks) return;
+    const long long base = (long long)chunk * kRxWaveCols + lane, c0 = base;
+    // wave-uniform: the butterflies below need every lane on the same path
+    const bool tail = (long long)(chunk + 1) * kRxWaveCols > n;
+    double xv[kRxLaneCols];
+    if (tail) {
+#pragma unroll
+        for (int q = 0; q < kRxLaneCols; ++q) {
+            const long long c = base + q * kWave;
+            xv[q] = c < n ? (double)x[c] : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kRxLaneCols; ++q) xv[q] = (double)x[base + q * kWave];
+    }
+    int j = 0;
+    auto emit = [&](const double (&s4)[4], int j0) {
+        const double tot = wave_sum4(s4, lane);
+        const int r = lane >> 4;
+        if ((lane & 15) == 0 && j0 + r < k) partials[(size_t)(j0 + r) * nchunks + chunk] = tot;
+    };
+    if (!tail) {
+        for (; j + 8 <= k; j += 8) {
+            double s[8];
+            rx_rows<8>(R, (size_t)n, j, c0, xv, s);
+            emit({s[0], s[1], s[2], s[3]}, j);
+            emit({s[4], s[5], s[6], s[7]}, j + 4);
+        }
+        for (; j < k; j += 4) {
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int a = 0; a < 4 && j + a < k; ++a) {
+                double s1[1];
+                rx_rows<1>(R, (size_t)n, j + a, c0, xv, s1);
+                s4[a] = s1[0];
+            }
+            emit(s4, j);
+        }
+    } else {
+        for (; j < k; j += 4) {
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int a = 0; a < 4 && j + a < k; ++a) {
+                const V *row = R + (size_t)(j + a) * n;
+                double t = 0.0;
+#pragma unroll
+                for (int q = 0; q < kRxLaneCols; ++q) {
+                    const long long c = base + q * kWave;
+                    if (c < n) t += (double)row[c] * xv[q];
+                }
+                s4[a] = t;
+            }
+            emit(s4, j);
+        }
+    }
+}
+
+// Pass 2: workgroup j folds row j's partials in chunk order into rx[j]; the
+// last workgroup to finish applies D and writes u = D R x (FP64).
 // D: dkind 1 diagonal, 2 dense (symmetric part taken from the upper triangle
 // of the row-major array, as the reference's lower-fill Dsymv on the
 // column-major view does, reference pdhg_core_op.cu:272-303).
-template <class V, class X, bool Gated>
-__global__ void __launch_bounds__(kBlock) rx_dense_kernel(const V *R, const X *x, int n, int k, double *partials,
-                                                           unsigned *counter, double *u, const double *ddiag,
-                                                           const double *ddense, int dkind, const int *gate) {
+template <bool Gated>
+__global__ void __launch_bounds__(kBlock) rx_fold_kernel(const double *partials, int nchunks, int k, double *rx,
+                                                          unsigned *counter, double *u, const double *ddiag,
+                                                          const double *ddense, int dkind, const int *gate) {
     if (Gated && *gate == 0) return;
-    __shared__ double xs[kRxChunk];
-    __shared__ double rx[kRxMaxRank];
+    __shared__ double lds[16];
+    __shared__ double rxs[kRxMaxRank];
     __shared__ int last;
-    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const long long c0 = (long long)blockIdx.x * kRxChunk;
-    const int len = (int)((n - c0) < kRxChunk ? (n - c0) : kRxChunk);
-    for (int t = threadIdx.x; t < len; t += kBlock) xs[t] = (double)x[c0 + t];
-    __syncthreads();
-    for (int j = wave; j < k; j += kBlock / kWave) {
-        const V *row = R + (size_t)j * n + c0;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int t = lane;
-        for (; t + 3 * kWave < len; t += 4 * kWave) {
-            s0 += (double)row[t] * xs[t];
-            s1 += (double)row[t + kWave] * xs[t + kWave];
-            s2 += (double)row[t + 2 * kWave] * xs[t + 2 * kWave];
-            s3 += (double)row[t + 3 * kWave] * xs[t + 3 * kWave];
-        }
-        for (; t < len; t += kWave) s0 += (double)row[t] * xs[t];
-        double s = wave_sum((s0 + s1) + (s2 + s3));
-        if (lane == 0) partials[(size_t)j * gridDim.x + blockIdx.x] = s;
+    const int j = blockIdx.x;
+    const double *P = partials + (size_t)j * nchunks;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};  // four loads in flight, fixed order
+    int c = threadIdx.x;
+    for (; c + 3 * kBlock < nchunks; c += 4 * kBlock)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] += P[c + q * kBlock];
+    for (; c < nchunks; c += kBlock) a[0] += P[c];
+    double v[1] = {(a[0] + a[1]) + (a[2] + a[3])};
+    block_reduce<1, 0>(v, lds);
+    if (threadIdx.x == 0) {
+        rx[j] = v[0];
+        __threadfence();
+        last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1 : 0;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1 : 0;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // fold: one wave per row j, lanes stride over the workgroups (coalesced)
-    for (int j = wave; j < k; j += kBlock / kWave) {
-        const double *P = partials + (size_t)j * gridDim.x;
-        double a = 0.0;
-        for (int b = lane; b < (int)gridDim.x; b += kWave) a += coherent_load(P + b);
-        a = wave_sum(a);
-        if (lane == 0) rx[j] = a;
-    }
+    for (int i = threadIdx.x; i < k; i += blockDim.x) rxs[i] = coherent_load(rx + i);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += blockDim.x) {
         double r;
-        if (dkind == 1) r = rx[i] * ddiag[i];
+        if (dkind == 1) r = rxs[i] * ddiag[i];
         else if (dkind == 2) {
             r = 0.0;
-            for (int j = 0; j < k; ++j) {
-                const int a = i < j ? i : j, b = i < j ? j : i;
-                r += ddense[(size_t)a * k + b] * rx[j];
+            for (int jj = 0; jj < k; ++jj) {
+                const int a = i < jj ? i : jj, b = i < jj ? jj : i;
+                r += ddense[(size_t)a * k + b] * rxs[jj];
             }
-        } else r = rx[i];
+        } else r = rxs[i];
         u[i] = r;
     }
     if (threadIdx.x == 0) *counter = 0u;
@@ -434,9 +528,18 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
                                                             const X *qdiag, const X *x, int accumulate,
                                                             const int *gate) {
     if (Gated && *gate == 0) return;
+    constexpr int JU = 10;  // rows of R loaded before the multiply-adds
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < k; ++j) s += (double)R[(size_t)j * n + i] * u[j];
+        int j = 0;
+        for (; j + JU <= k; j += JU) {
+            double r[JU];
+#pragma unroll
+            for (int a = 0; a < JU; ++a) r[a] = (double)R[(size_t)(j + a) * n + i];
+#pragma unroll
+            for (int a = 0; a < JU; ++a) s += r[a] * u[j + a];
+        }
+        for (; j < k; ++j) s += (double)R[(size_t)j * n + i] * u[j];
         if (qdiag) s = s + (double)qdiag[i] * (double)x[i];
         else if (accumulate) s = s + (double)qx[i];
         qx[i] = (X)s;
@@ -482,7 +585,7 @@ __global__ void __launch_bounds__(kBlock) residual_rows_kernel(const X *ax, cons
         v[1] += fmax(y, 0.0) * (double)cl_fin[i] + fmin(y, 0.0) * (double)cu_fin[i] - (double)g[i] * y;
         v[2] = fmax(v[2], abs_or_inf(pres));
     }
-    grid_reduce<2, 1>(gr, v);
+    grid_partials<2, 1>(gr, v);
 }
 
 // Columns: dual residual, dual slack and objective terms.  qp = 0 uses the
@@ -522,7 +625,7 @@ __global__ void __launch_bounds__(kBlock) residual_cols_kernel(const X *c, const
         v[7] = fmax(v[7], abs_or_inf(qs));
         v[8] = fmax(v[8], abs_or_inf(as));
     }
-    grid_reduce<6, 3>(gr, v);
+    grid_partials<6, 3>(gr, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -551,7 +654,7 @@ __global__ void __launch_bounds__(kBlock) ray_project_kernel(X *dx, const X *lb,
             v[1] = fmax(v[1], abs_or_inf(d));
         }
     }
-    grid_reduce<0, 2>(gr, v);
+    grid_partials<0, 2>(gr, v);
 }
 
 // out = [sum dual-ray row terms, max primal infeasibility]
@@ -568,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) ray_rows_kernel(const X *dy, const X *
                             fmax(0.0, pv) * (double)isfinite((double)cu[i])) * con_rescale[i];
         v[1] = fmax(v[1], abs_or_inf(inf));
     }
-    grid_reduce<1, 1>(gr, v);
+    grid_partials<1, 1>(gr, v);
 }
 
 // out = [c'dx, sum dual-slack ray terms, max dual infeasibility]
@@ -585,7 +688,7 @@ __global__ void __launch_bounds__(kBlock) ray_cols_kernel(const X *c, const X *d
                             fmin(0.0, na) * (double)!isfinite((double)ub[i])) * var_rescale[i];
         v[2] = fmax(v[2], abs_or_inf(inf));
     }
-    grid_reduce<2, 1>(gr, v);
+    grid_partials<2, 1>(gr, v);
 }
 
 // out = [max |a|]
@@ -594,7 +697,7 @@ __global__ void __launch_bounds__(kBlock) inf_norm_kernel(const X *a, int n, Gri
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         v[0] = fmax(v[0], abs_or_inf((double)a[i]));
-    grid_reduce<0, 1>(gr, v);
+    grid_partials<0, 1>(gr, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -637,8 +740,12 @@ __global__ void __launch_bounds__(kBlock) pi_dot_kernel(const X *ne, const X *e,
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
         v[0] += (double)ne[i] * (double)e[i];
-    if (grid_reduce<1, 0>(gr, v) && threadIdx.x == 0) ds->pi_value = v[0];
+    grid_partials<1, 0>(gr, v);
 }
+
+struct TailPiValue {
+    __device__ void operator()(const double *r, DevScalars *ds) const { ds->pi_value = r[0]; }
+};
 
 template <class X>
 __global__ void __launch_bounds__(kBlock) pi_residual_kernel(X *ne, const X *e, int m, double tol, GridRed gr,
@@ -651,10 +758,15 @@ __global__ void __launch_bounds__(kBlock) pi_residual_kernel(X *ne, const X *e, 
         ne[i] = (X)t;
         v[0] += t * t;
     }
-    if (grid_reduce<1, 0>(gr, v) && threadIdx.x == 0) {
-        if (sqrt(v[0]) < tol) ds->pi_active = 0;
-    }
+    grid_partials<1, 0>(gr, v);
 }
+
+struct TailPiResidual {  // stop when the eigen-residual is below tol
+    double tol;
+    __device__ void operator()(const double *r, DevScalars *ds) const {
+        if (sqrt(r[0]) < tol) ds->pi_active = 0;
+    }
+};
 
 template <class X>
 __global__ void __launch_bounds__(kBlock) pi_restore_kernel(X *ne, const X *e, int m, GridRed gr, DevScalars *ds) {
@@ -666,12 +778,16 @@ __global__ void __launch_bounds__(kBlock) pi_restore_kernel(X *ne, const X *e, i
         ne[i] = (X)t;
         v[0] += t * t;
     }
-    if (grid_reduce<1, 0>(gr, v) && threadIdx.x == 0) {
-        const double nrm = sqrt(v[0]);
+    grid_partials<1, 0>(gr, v);
+}
+
+struct TailPiRestore {
+    __device__ void operator()(const double *r, DevScalars *ds) const {
+        const double nrm = sqrt(r[0]);
         ds->pi_inv = 1.0 / sqrt(nrm * nrm);
         ds->pi_iter += 1;
     }
-}
+};
 
 // Eigenvalue power iteration pieces: inv = 1/||v||.
 template <class X>
@@ -680,8 +796,12 @@ __global__ void __launch_bounds__(kBlock) pi_norm_kernel(const X *a, int n, Grid
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         v[0] += (double)a[i] * (double)a[i];
-    if (grid_reduce<1, 0>(gr, v) && threadIdx.x == 0) ds->pi_inv = 1.0 / sqrt(v[0]);
+    grid_partials<1, 0>(gr, v);
 }
+
+struct TailPiNorm {
+    __device__ void operator()(const double *r, DevScalars *ds) const { ds->pi_inv = 1.0 / sqrt(r[0]); }
+};
 
 // shifted = -av + shift*v (shift = 0 means shifted = av); lambda = v'shifted;
 // stop when |lambda - old| < tol after the first iteration; else v = shifted.
@@ -698,14 +818,19 @@ __global__ void __launch_bounds__(kBlock) pi_rayleigh_kernel(const X *v_in, X *a
         }
         v[0] += (double)v_in[i] * t;
     }
-    if (grid_reduce<1, 0>(gr, v) && threadIdx.x == 0) {
+    grid_partials<1, 0>(gr, v);
+}
+
+struct TailPiRayleigh {  // stop when the Rayleigh quotient settles
+    double tol;
+    __device__ void operator()(const double *r, DevScalars *ds) const {
         const double old = ds->pi_value;
         ds->pi_old = old;
-        ds->pi_value = v[0];
+        ds->pi_value = r[0];
         const int it = ds->pi_iter;
         ds->pi_iter = it + 1;
-        if (it > 0 && fabs(v[0] - old) < tol) ds->pi_active = 0;
+        if (it > 0 && fabs(r[0] - old) < tol) ds->pi_active = 0;
     }
-}
+};
 
 }  // namespace pdhcg

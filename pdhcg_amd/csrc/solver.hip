@@ -193,6 +193,23 @@ class DevConeSet {
                                gate);
     }
 
+    // Cone step of the diagonal-Q prox: projection onto diag(d) K in the
+    // metric diag(1 + tau q) (PSD blocks excluded; they take the BB path).
+    template <class X> void project_diag_q(X *v, const double *d, const X *q, double tau, hipStream_t st) const {
+        if (rsoc.count)
+            hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0,
+                               st, v, d, q, tau, rsoc);
+        if (soc.count)
+            hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_STANDARD_SOC>), blocks(soc.count), dim3(kBlock), 0,
+                               st, v, d, q, tau, soc);
+        if (expc.count)
+            hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_EXPONENTIAL>), blocks(expc.count), dim3(kBlock), 0,
+                               st, v, d, q, tau, expc);
+        if (powc.count)
+            hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v,
+                               d, q, tau, powc);
+    }
+
     // Dual-cone residuals (rc - P_{K*}(rc)) d of rc = c + qx - aty (qx may be
     // null) written over dres on every cone slot; scratch (n entries) holds the
     // PSD projections (reference src/cone_dispatch.cu:88-115).
@@ -286,12 +303,21 @@ template <class V, class X> class Solver final : public SolverBase {
     double power_eig(bool minimum, double shift);
 
     // ---- device operations
-    GridRed red() const { return GridRed{red_partials_, red_counter_, ds_->red}; }
+    GridRed red(int offset = 0) const { return GridRed{red_partials_, ds_->red + offset}; }
+    // Folds the partials of the reducing launch just enqueued (grid of
+    // `blocks`) into ds_->red[offset...] and runs its tail on the device.
+    template <int NS, int NM, class Tail = NoTail>
+    void fold(int blocks, int offset = 0, Tail tail = Tail(), const int *gate = nullptr) {
+        hipLaunchKernelGGL((grid_fold_kernel<NS, NM, Tail, DevScalars>), dim3(1), dim3(kBlock), 0, st_, red(offset),
+                           blocks, tail, ds_, gate);
+    }
     void read_scalars() {
         HIP_OK(hipMemcpyAsync(hs_, ds_, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
         HIP_OK(hipStreamSynchronize(st_));
     }
     int grid(long long n) const { return blocks_for(n); }
+    // one thread per element, no cap (kernels without a grid reduction)
+    static int full_grid(long long n) { return (int)std::max<long long>(1, (n + kBlock - 1) / kBlock); }
     // LP and diagonal-Q primal steps are closed form; the rest use BB
     bool closed_prox() const { return qtype_ == PDHCG_NON_Q || (qtype_ == PDHCG_DIAG_Q && !has_psd_); }
     void q_product(const X *x, X *qx, const int *gate);
@@ -324,8 +350,9 @@ template <class V, class X> class Solver final : public SolverBase {
     V *Rd_ = nullptr;                // dense R, k x n row-major
     bool r_dense_ = false;
     int dkind_ = 0;
-    double *ddiag_ = nullptr, *ddense_ = nullptr, *u_ = nullptr, *rx_partials_ = nullptr;
+    double *ddiag_ = nullptr, *ddense_ = nullptr, *u_ = nullptr, *rx_partials_ = nullptr, *rx_sums_ = nullptr;
     unsigned *rx_counter_ = nullptr;
+    int rx_chunks_ = 0;
     X *rxv_ = nullptr;
     int rx_blocks_ = 0;
 
@@ -357,7 +384,6 @@ template <class V, class X> class Solver final : public SolverBase {
 
     // reductions and device scalars
     double *red_partials_ = nullptr;
-    unsigned *red_counter_ = nullptr;
     DevScalars *ds_ = nullptr, *hs_ = nullptr;
 
     // outer-loop state (reference internal/internal_types.h:137-236)
@@ -376,11 +402,6 @@ template <class V, class X> class Solver final : public SolverBase {
 
 template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params) : P_(params) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (sp.qtype == PDHCG_DIAG_Q && sp.cones && sp.cones->num_cones > 0) {
-        bool psd = false;
-        for (int k = 0; k < sp.cones->num_cones; ++k) psd |= sp.cones->type[k] == CONE_PSD;
-        if (!psd) fail("the diagonal-Q proximal step with cone blocks is not supported");
-    }
     HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     n_ = sp.n;
     m_ = sp.m;
@@ -393,8 +414,7 @@ template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const 
     obj_norm_ = sp.obj_norm;
     bound_norm_ = sp.bound_norm;
     rescale_time_ = sp.rescale_time;
-    red_partials_ = dzeros<double>((size_t)kMaxRedBlocks * kMaxRedValues);
-    red_counter_ = dzeros<unsigned>(1);
+    red_partials_ = dzeros<double>(kRedPartials);
     ds_ = dzeros<DevScalars>(1);
     HIP_OK(hipHostMalloc((void **)&hs_, sizeof(DevScalars), hipHostMallocDefault));
     std::memset(hs_, 0, sizeof(DevScalars));
@@ -465,8 +485,10 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
             for (int j = 0; j < k_; ++j)
                 for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e) Rd[(size_t)j * n_ + sp.R.ci[e]] += sp.R.v[e];
             Rd_ = dupload<V>(Rd, Rd.size());
-            rx_blocks_ = (int)std::max<long long>(1, ((long long)n_ + kRxChunk - 1) / kRxChunk);
-            rx_partials_ = dzeros<double>((size_t)rx_blocks_ * k_);
+            rx_chunks_ = rx_chunks(n_);
+            rx_blocks_ = std::max(1, (rx_chunks_ + kBlock / kWave - 1) / (kBlock / kWave));
+            rx_partials_ = dzeros<double>((size_t)rx_chunks_ * k_);
+            rx_sums_ = dzeros<double>((size_t)k_);
             rx_counter_ = dzeros<unsigned>(1);
         } else {
             Rc_ = upload_csr<V>(sp.R);
@@ -624,12 +646,12 @@ template <class V, class X> Solver<V, X>::~Solver() {
     free_spmv_plan(pQ_);
     free_spmv_plan(pR_);
     free_spmv_plan(pRt_);
-    void *ptrs[] = {qsdiag_, qdiag_, Rd_,     ddiag_,  ddense_,      u_,          rx_partials_, rx_counter_,
+    void *ptrs[] = {qsdiag_, qdiag_, Rd_,     ddiag_,  ddense_,      u_,          rx_partials_, rx_counter_, rx_sums_,
                     rxv_,    lb_,    ub_,     c_,      cl_,          cu_,         g_,           cl_fin_,
                     cu_fin_, lb_fin_, ub_fin_, con_rescale_, var_rescale_, x0_,     x_,           xp_,
                     xr_,     y0_,    y_,      yp_,     yr_,          aty_,        ax_,          dslack_,
                     dx_,     dy_,    qx_,     grad_,   dirn_,        mdiag_,      minv_,        diag_h_,
-                    red_partials_, red_counter_, ds_, snap_, pgp_, dres_, aff_d_, acomp_, aproj_, apres_, amem_};
+                    red_partials_, ds_, snap_, pgp_, dres_, aff_d_, acomp_, aproj_, apres_, amem_};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     cones_.release();
@@ -663,9 +685,12 @@ template <class V, class X> double Solver<V, X>::power_sv() {
             spmv_launch<V, X, Acc>(At_, pAt_, ev, EpiStore<X>{dp}, st_, gate);
             spmv_launch<V, X, Acc>(A_, pA_, dp, EpiStore<X>{ne}, st_, gate);
             hipLaunchKernelGGL(pi_dot_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, red(), ds_);
+            fold<1, 0>(grid(m_), 0, TailPiValue{}, gate);
             hipLaunchKernelGGL(pi_residual_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, P_.sv_tol,
                                red(), ds_);
+            fold<1, 0>(grid(m_), 0, TailPiResidual{P_.sv_tol}, gate);
             hipLaunchKernelGGL(pi_restore_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, red(), ds_);
+            fold<1, 0>(grid(m_), 0, TailPiRestore{}, gate);
             hipLaunchKernelGGL(scale_dev_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, &ds_->pi_inv, m_, gate);
         }
         HIP_OK(hipGetLastError());
@@ -698,10 +723,12 @@ template <class V, class X> double Solver<V, X>::power_eig(bool minimum, double 
         const int todo = std::min(chunk, P_.sv_max_iter - it);
         for (int c = 0; c < todo; ++c, ++it) {
             hipLaunchKernelGGL(pi_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, n_, red(), ds_);
+            fold<1, 0>(grid(n_), 0, TailPiNorm{}, gate);
             hipLaunchKernelGGL(scale_dev_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, &ds_->pi_inv, n_, gate);
             spmv_launch<V, X, Acc>(Q_, pQ_, v, EpiStore<X>{av}, st_, gate);
             hipLaunchKernelGGL(pi_rayleigh_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, av, shift,
                                minimum ? 1 : 0, n_, P_.sv_tol, red(), ds_);
+            fold<1, 0>(grid(n_), 0, TailPiRayleigh{P_.sv_tol}, gate);
             hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, av, n_, gate);
         }
         HIP_OK(hipGetLastError());
@@ -737,18 +764,22 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
             const X *qd = (plus && !q_sparse_) ? qsdiag_ : nullptr;
             const int acc = (plus && q_sparse_) ? 1 : 0;
             if (gate) {
-                hipLaunchKernelGGL((rx_dense_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_,
-                                   k_, rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
-                hipLaunchKernelGGL((rtu_dense_kernel<V, X, true>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_, qx,
-                                   qd, x, acc, gate);
+                hipLaunchKernelGGL((rx_partial_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_,
+                                   k_, rx_partials_, rx_chunks_, gate);
+                hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
+                                   rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, true>), dim3(full_grid(n_)), dim3(kBlock), 0, st_, Rd_, u_,
+                                   n_, k_, qx, qd, x, acc, gate);
             } else {
                 const int t0 = timer_.start(kTimedRx, st_);
-                hipLaunchKernelGGL((rx_dense_kernel<V, X, false>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x, n_,
-                                   k_, rx_partials_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+                hipLaunchKernelGGL((rx_partial_kernel<V, X, false>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, x,
+                                   n_, k_, rx_partials_, rx_chunks_, gate);
                 timer_.stop(kTimedRx, t0, st_);
+                hipLaunchKernelGGL(rx_fold_kernel<false>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
+                                   rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
                 const int t1 = timer_.start(kTimedRtu, st_);
-                hipLaunchKernelGGL((rtu_dense_kernel<V, X, false>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_,
-                                   qx, qd, x, acc, gate);
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, false>), dim3(full_grid(n_)), dim3(kBlock), 0, st_, Rd_,
+                                   u_, n_, k_, qx, qd, x, acc, gate);
                 timer_.stop(kTimedRtu, t1, st_);
             }
         } else {
@@ -774,7 +805,8 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     const int nb = grid(n_);
     if (precond_ && cached_inv_tau_ != inv) {
         hipLaunchKernelGGL(bb_refresh_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, diag_h_, inv, mdiag_, minv_, n_,
-                           red(), ds_);
+                           red());
+        fold<1, 0>(nb, 0, TailTolScale{n_});
         cached_inv_tau_ = inv;
     }
     const X *mi = precond_ ? minv_ : nullptr;
@@ -783,13 +815,15 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     // With cones every box step is followed by the cone projection and the
     // step is re-measured from the projected point (reference :561-566,652-657).
     const int defer = has_cones_ ? 1 : 0;
+    const int pre = precond_ ? 1 : 0;
     hipLaunchKernelGGL(bb_init_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, aty_, grad_, dirn_, x_, xp_, c_, qx_, lb_,
-                       ub_, mi, md, alpha0, n_, red(), ds_, inner_tol_, inner_limit_, defer);
+                       ub_, mi, md, alpha0, n_, red(), ds_, defer);
     if (has_cones_) {
         project_cones(xp_, false, nullptr);
         hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, dirn_, md, n_, 1, red(),
-                           ds_, inner_tol_, inner_limit_);
+                           ds_);
     }
+    fold<1, 0>(nb, 0, TailBBStep{1, pre, inner_limit_, inner_tol_});
     const int *gate = &ds_->bb_active;
     const int max_bodies = inner_limit_ - 1;
     int issued = 0, chunk = 2;
@@ -798,16 +832,18 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
         for (int c = 0; c < todo; ++c) {
             q_product(xp_, qx_, gate);
             hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_, grad_,
-                               dirn_, inv, n_, precond_ ? 1 : 0, red(), ds_);
+                               dirn_, inv, n_, red(), ds_);
+            fold<1, 0>(nb, 0, TailBBAlpha{pre}, gate);
             if (has_cones_)
                 hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, snap_, xp_, n_, gate);
             hipLaunchKernelGGL(bb_direction_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, grad_, dirn_, lb_, ub_,
-                               mi, md, n_, red(), ds_, inner_tol_, inner_limit_, defer);
+                               mi, md, n_, red(), ds_, defer);
             if (has_cones_) {
                 project_cones(xp_, false, gate);
                 hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, snap_, dirn_, md, n_,
-                                   0, red(), ds_, inner_tol_, inner_limit_);
+                                   0, red(), ds_);
             }
+            fold<1, 0>(nb, 0, TailBBStep{0, pre, inner_limit_, inner_tol_}, gate);
         }
         issued += todo;
         HIP_OK(hipGetLastError());
@@ -843,7 +879,8 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
         spmv_launch<V, X, Acc>(At_, pAt_, y_, epi, st_);
         timer_.stop(kTimedPrimalStep, t, st_);
         if (cones) {
-            project_cones(xp_, false, nullptr);
+            if (qtype_ == PDHCG_DIAG_Q) cones_.project_diag_q(xp_, var_rescale_, qdiag_, tau, st_);
+            else project_cones(xp_, false, nullptr);
             hipLaunchKernelGGL(cone_reflect_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, xr_, xp_, x_, cones_.mask,
                                n_);
         }
@@ -881,10 +918,12 @@ template <class V, class X> void Solver<V, X>::halpern(double w) {
 template <class V, class X> void Solver<V, X>::fixed_point_error() {
     hipLaunchKernelGGL(delta_norms_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xr_, x_, dx_, yr_,
                        y_, dy_, n_, m_, red());
+    fold<2, 0>(grid((long long)n_ + m_));
     read_scalars();
     const double pn2 = hs_->red[0], dn2 = hs_->red[1];
     spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
     hipLaunchKernelGGL(dot_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, aty_, dx_, n_, red(), nullptr);
+    fold<1, 0>(grid(n_));
     read_scalars();
     const double cross = hs_->red[0];
     const double pn = std::sqrt(pn2), dn = std::sqrt(dn2);
@@ -898,6 +937,7 @@ template <class V, class X> void Solver<V, X>::restart() {
     const restart_parameters_t &rp = P_.restart_params;
     hipLaunchKernelGGL(delta_norms_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, x0_, dx_, yp_,
                        y0_, dy_, n_, m_, red());
+    fold<2, 0>(grid((long long)n_ + m_));
     read_scalars();
     const double pd = std::sqrt(hs_->red[0]), dd = std::sqrt(hs_->red[1]);
     const double ratio = rel_du_ / rel_pr_;
@@ -948,33 +988,30 @@ template <class V, class X> void Solver<V, X>::residual() {
     spmv_launch<V, X, Acc>(A_, pA_, xp_, EpiStore<X>{ax_}, st_);
     spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
     q_product(xp_, qx_, nullptr);
-    GridRed r0 = red();
+    const GridRed r0 = red();
     if (has_affine_) {
         // red[1] row terms, [20..23] rows (sum pres^2, sum amem^2, max pres, max amem), [24..25] complementarity
-        GridRed ra = r0;
-        ra.out = ds_->red + 1;
         hipLaunchKernelGGL(affine_rows_prepare_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_,
-                           yp_, cl_fin_, cu_fin_, apres_, amem_, m_, ra);
+                           yp_, cl_fin_, cu_fin_, apres_, amem_, m_, r0);
+        fold<1, 0>(grid(m_), 1);
         acones_.project(apres_, aff_d_, true, nullptr, st_);
         const int waves = kBlock / kWave;
         hipLaunchKernelGGL(affine_complementarity_kernel<X>, dim3((acones_.total + waves - 1) / waves), dim3(kBlock), 0,
                            st_, ax_, g_, yp_, amem_, acones_.all_start, acones_.all_len, acones_.total, cbr_, acomp_);
         acones_.project(amem_, aff_d_, true, nullptr, st_);
-        GridRed rb = r0;
-        rb.out = ds_->red + 20;
         hipLaunchKernelGGL(affine_rows_finish_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, apres_, amem_,
-                           con_rescale_, aff_d_, m_, rb);
-        GridRed rc = r0;
-        rc.out = ds_->red + 24;
-        hipLaunchKernelGGL(sq_max_kernel, dim3(grid(acones_.total)), dim3(kBlock), 0, st_, acomp_, acones_.total, rc);
+                           con_rescale_, aff_d_, m_, r0);
+        fold<2, 2>(grid(m_), 20);
+        hipLaunchKernelGGL(sq_max_kernel, dim3(grid(acones_.total)), dim3(kBlock), 0, st_, acomp_, acones_.total, r0);
+        fold<1, 1>(grid(acones_.total), 24);
     } else {
         hipLaunchKernelGGL(residual_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ax_, g_, cl_, cu_, yp_,
                            con_rescale_, cl_fin_, cu_fin_, m_, r0);
+        fold<2, 1>(grid(m_), 0);
     }
-    GridRed r1 = r0;
-    r1.out = ds_->red + 3;
     hipLaunchKernelGGL(residual_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, aty_, qx_, xp_, lb_, ub_,
-                       dslack_, var_rescale_, tau, is_lp_ ? 0 : 1, n_, has_cones_ ? dres_ : nullptr, r1);
+                       dslack_, var_rescale_, tau, is_lp_ ? 0 : 1, n_, has_cones_ ? dres_ : nullptr, r0);
+    fold<6, 3>(grid(n_), 3);
     if (has_cones_) {
         // red[12..13] augmented dual residual, [14..17] power violation, [18] cone slack correction
         const X *qx = is_lp_ ? nullptr : qx_;
@@ -985,20 +1022,17 @@ template <class V, class X> void Solver<V, X>::residual() {
         hipLaunchKernelGGL(projected_gradient_point_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, pgp_, xp_, c_, qx,
                            aty_, lb_, ub_, t, n_);
         project_cones(pgp_, true, nullptr);
-        GridRed r2 = r0;
-        r2.out = ds_->red + 12;
         hipLaunchKernelGGL(augment_dual_residual_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, dres_, xp_, pgp_,
-                           var_rescale_, t, n_, r2);
+                           var_rescale_, t, n_, r0);
+        fold<1, 1>(grid(n_), 12);
         if (has_power_) {
-            GridRed r3 = r0;
-            r3.out = ds_->red + 14;
             hipLaunchKernelGGL(power_violation_kernel<X>, dim3(grid(cones_.powc.count)), dim3(kBlock), 0, st_, xp_,
-                               var_rescale_, cones_.powc, cbr_, r3);
+                               var_rescale_, cones_.powc, cbr_, r0);
+            fold<2, 2>(grid(cones_.powc.count), 14);
         }
-        GridRed r4 = r0;
-        r4.out = ds_->red + 18;
         hipLaunchKernelGGL(cone_slack_correction_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, qx, aty_,
-                           dslack_, xp_, cones_.mask, n_, r4);
+                           dslack_, xp_, cones_.mask, n_, r0);
+        fold<1, 0>(grid(n_), 18);
     }
     HIP_OK(hipGetLastError());
     read_scalars();
@@ -1040,25 +1074,25 @@ template <class V, class X> void Solver<V, X>::residual() {
 template <class V, class X> void Solver<V, X>::infeasibility() {
     hipLaunchKernelGGL(ray_project_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, dx_, lb_, ub_, dy_,
                        cl_, cu_, n_, m_, red());
+    fold<0, 2>(grid((long long)n_ + m_));
     read_scalars();
     const double prn = hs_->red[0], drn = hs_->red[1];
     if (prn > 0.0) hipLaunchKernelGGL(scale_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, dx_, 1.0 / prn, n_);
     spmv_launch<V, X, Acc>(A_, pA_, dx_, EpiStore<X>{ax_}, st_);
     spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
-    GridRed r0 = red();
+    const GridRed r0 = red();
     hipLaunchKernelGGL(ray_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, dy_, ax_, g_, cl_, cu_, cl_fin_,
                        cu_fin_, con_rescale_, m_, r0);
-    GridRed r1 = r0;
-    r1.out = ds_->red + 2;
+    fold<1, 1>(grid(m_), 0);
     hipLaunchKernelGGL(ray_cols_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, dx_, aty_, lb_, ub_, lb_fin_,
-                       ub_fin_, var_rescale_, n_, r1);
+                       ub_fin_, var_rescale_, n_, r0);
+    fold<2, 1>(grid(n_), 2);
     double qn = 0.0;
     const bool has_q = !is_lp_ && qtype_ != PDHCG_NON_Q;
     if (has_q) {
         q_product(dx_, qx_, nullptr);
-        GridRed r2 = r0;
-        r2.out = ds_->red + 5;
-        hipLaunchKernelGGL(inf_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, qx_, n_, r2);
+        hipLaunchKernelGGL(inf_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, qx_, n_, r0);
+        fold<0, 1>(grid(n_), 5);
     }
     HIP_OK(hipGetLastError());
     read_scalars();

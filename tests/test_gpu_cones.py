@@ -23,8 +23,8 @@ import scipy.sparse as sp
 import pdhcg_amd
 from pdhcg_amd import QPData, capi
 from tests.oracle_lib import oracle_solve
-from tests.test_oracle_cones import FREE, FREE_UB, _L, _mat_to_svec, in_exp, project as orc_project, \
-    sparse_q_soc_problem
+from tests.test_oracle_cones import FREE, FREE_UB, _L, _mat_to_svec, diag_q_cone_problem, in_exp, \
+    project as orc_project, sparse_q_soc_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -281,6 +281,23 @@ def test_diag_q_with_psd_uses_inner_solve():
 
 
 def test_unsupported_cone_configurations_are_rejected():
-    p = QPData(c=[1.0, 0.0, 0.0], Q=sp.diags([1.0, 1.0, 1.0]).tocsr(), A=sp.csr_matrix([[1.0, 0.0, 0.0]]),
-               con_lb=[1.0], con_ub=[1.0], var_lb=FREE, var_ub=FREE_UB, cones=[(SOC, 0, 1)])
+    # second-order blocks longer than the per-thread limit (64 vector slots)
+    n = 67
+    p = QPData(c=np.zeros(n), A=sp.csr_matrix(np.eye(1, n)), con_lb=[1.0], con_ub=[1.0], var_lb=[-1e30] * n,
+               var_ub=[1e30] * n, cones=[(SOC, 0, 65)])
     assert pdhcg_amd.solve(p, eps=1e-6) is None
+
+
+@pytest.mark.parametrize("cone_type", [SOC, RSOC, EXP, POW])
+def test_diag_q_cone_prox_matches_oracle(cone_type):
+    # diagonal Q with a cone block: the cone step is a projection in the
+    # metric diag(1 + tau q)
+    p, _ = diag_q_cone_problem(11 + cone_type, cone_type)
+    _device_vs_oracle(p, 1e-7, iteration_limit=500000)
+
+
+def test_random_conic_diag_qp_matches_oracle():
+    def q(n, rng):
+        return sp.diags(rng.uniform(0.1, 2.0, n)).tocsr()
+    p = random_conic(59, (SOC, RSOC, POW), q=q)
+    _device_vs_oracle(p, 1e-6, iteration_limit=200000)

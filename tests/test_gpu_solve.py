@@ -70,6 +70,14 @@ def test_random_matches_oracle(kind, seed):
     _check_pair(p, 1e-6)
 
 
+@pytest.mark.parametrize("n", [193, 200, 256, 257, 700, 1023])
+def test_lowrank_column_chunking(n):
+    # the dense R x pass splits the columns into 256-wide chunks; sizes around
+    # and off that width exercise the partial last chunk
+    p = workloads.random_problem("lowrank_diag", seed=n, n=n, m=max(10, n // 4))
+    _check_pair(p, 1e-6)
+
+
 def test_sparse_lowrank_csr_path():
     p = workloads.random_problem("lowrank_sparse_r", seed=3, n=1000, m=150)
     _check_pair(p, 1e-4)

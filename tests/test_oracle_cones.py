@@ -357,3 +357,68 @@ def test_affine_soc_portfolio_matches_slsqp(n, k, seed):
     assert np.sqrt(te2(o.x)) >= 0.999 * tau           # the cone constraint is active
     assert abs(o.pobj - r.fun) <= 1e-7 * (1 + abs(r.fun)), (o.pobj, r.fun)
     assert np.max(np.abs(o.x - r.x)) <= 1e-5
+
+
+def diag_q_cone_problem(seed, cone_type):
+    """min 1/2 x' diag(q) x + c'x  s.t.  A x = b, x in K on a block, 0 <= x <= 2
+    elsewhere; small, strictly feasible, for an independent solver."""
+    rng = np.random.default_rng(seed)
+    if cone_type == capi.CONE_STANDARD_SOC:
+        cone, L, xin = (cone_type, 0, 3), 5, np.array([0.3, -0.2, 0.1, 0.2, 1.0])
+    elif cone_type == capi.CONE_ROTATED_SOC:
+        cone, L, xin = (cone_type, 0, 2), 4, np.array([0.3, -0.2, 1.0, 1.0])
+    elif cone_type == capi.CONE_EXPONENTIAL:
+        cone, L, xin = (cone_type, 0, 1), 3, np.array([0.0, 1.0, 2.0])
+    else:
+        cone, L, xin = (cone_type, 0, 1, 0.4), 3, np.array([1.0, 1.0, 0.3])
+    n, m = L + 4, 3
+    x0 = np.concatenate([xin, rng.uniform(0.5, 1.5, n - L)])
+    A = rng.standard_normal((m, n))
+    lb, ub = np.zeros(n), np.full(n, 2.0)
+    lb[:L], ub[:L] = -np.inf, np.inf
+    c = rng.standard_normal(n)
+    if cone_type == capi.CONE_POWER:
+        c[2] = -3.0     # push |z| against the cone boundary,
+        A[:, 2] = 0.0   # with z left out of the equalities
+    return QPData(c=c, Q=sp.diags(rng.uniform(0.5, 2.0, n)).tocsr(), A=sp.csr_matrix(A),
+                  con_lb=A @ x0, con_ub=A @ x0, var_lb=lb, var_ub=ub, cones=[cone]), L
+
+
+def _cone_constraint(cone_type, L, alpha=0.4):
+    # smooth inequality forms g(x) >= 0 of the cone on x[:L]
+    if cone_type == capi.CONE_STANDARD_SOC:
+        return lambda x: x[L - 1] - np.sqrt(np.sum(x[:L - 1] ** 2) + 1e-300)
+    if cone_type == capi.CONE_ROTATED_SOC:
+        return lambda x: 2 * x[L - 2] * x[L - 1] - np.sum(x[:L - 2] ** 2)
+    if cone_type == capi.CONE_EXPONENTIAL:
+        return lambda x: x[2] - x[1] * np.exp(x[0] / x[1])
+    return lambda x: np.abs(x[0]) ** alpha * np.abs(x[1]) ** (1 - alpha) - abs(x[2])
+
+
+@pytest.mark.parametrize("cone_type", [capi.CONE_STANDARD_SOC, capi.CONE_ROTATED_SOC, capi.CONE_EXPONENTIAL,
+                                       capi.CONE_POWER])
+def test_diag_q_cone_prox_matches_slsqp(cone_type):
+    # the diagonal-Q cone step (weighted projection) against scipy's SLSQP on
+    # the same problem with the cone written as smooth inequalities
+    from scipy.optimize import minimize
+    p, L = diag_q_cone_problem(11 + cone_type, cone_type)
+    o = oracle_solve(p, eps=1e-8, iteration_limit=500000)
+    q, c, A, b = p.Q.diagonal(), np.asarray(p.c), p.A.toarray(), np.asarray(p.con_lb)
+    n = len(c)
+    cons = [{"type": "eq", "fun": lambda x: A @ x - b}, {"type": "ineq", "fun": _cone_constraint(cone_type, L)}]
+    if cone_type == capi.CONE_ROTATED_SOC:
+        cons.append({"type": "ineq", "fun": lambda x: x[L - 2:L]})
+    if cone_type == capi.CONE_EXPONENTIAL:
+        cons.append({"type": "ineq", "fun": lambda x: x[1] - 1e-9})
+    if cone_type == capi.CONE_POWER:
+        cons.append({"type": "ineq", "fun": lambda x: x[:2]})
+    bounds = [(None, None)] * L + [(0.0, 2.0)] * (n - L)
+    best = None
+    for start in (o.x, np.asarray(p.var_lb).clip(0, None) * 0 + 0.5):
+        r = minimize(lambda x: 0.5 * (q * x) @ x + c @ x, start, jac=lambda x: q * x + c, bounds=bounds,
+                     constraints=cons, method="SLSQP", options={"ftol": 1e-14, "maxiter": 3000})
+        if r.success and (best is None or r.fun < best.fun):
+            best = r
+    assert o.status == "OPTIMAL" and best is not None
+    assert abs(_cone_constraint(cone_type, L)(o.x)) <= 1e-6   # the cone is active at the optimum
+    assert abs(o.pobj - best.fun) <= 1e-6 * (1 + abs(best.fun)), (o.pobj, best.fun)
