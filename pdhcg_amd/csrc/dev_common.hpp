@@ -21,7 +21,7 @@ namespace pdhcg {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;            // threads per block of the elementwise/reduction kernels
-constexpr int kMaxRedBlocks = 1024;    // upper bound on blocks of any reducing launch
+constexpr int kMaxRedBlocks = 4096;    // upper bound on blocks of any reducing launch
 constexpr int kMaxRedValues = 16;      // values reduced by one launch
 constexpr size_t kRedPartials = (size_t)kMaxRedValues * kMaxRedBlocks;
 
@@ -101,8 +101,9 @@ struct NoTail {
 };
 
 // One workgroup folds the partial packs of `nblocks` workgroups in a fixed
-// order (thread t takes blocks t, t + 256, ... with four loads in flight),
-// writes g.out and hands the values to tail(values, scalars) in thread 0.
+// order (thread t takes blocks t, t + 256, ... into four accumulators, so four
+// loads are in flight), writes g.out and hands the values to
+// tail(values, scalars) in thread 0.
 // gate: skip everything when *gate == 0 (device-resident loops).
 template <int NS, int NM, class Tail, class S>
 __global__ void __launch_bounds__(kBlock) grid_fold_kernel(GridRed g, int nblocks, Tail tail, S *scalars,
@@ -113,12 +114,15 @@ __global__ void __launch_bounds__(kBlock) grid_fold_kernel(GridRed g, int nblock
     double t[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double a[4];
+        const double id = i < NS ? 0.0 : -INFINITY;
+        const double *P = g.partials + (size_t)i * kMaxRedBlocks;
+        double a[4] = {id, id, id, id};
+        for (int b0 = threadIdx.x; b0 < nblocks; b0 += 4 * kBlock)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int b = threadIdx.x + q * kBlock;
-            a[q] = b < nblocks ? g.partials[(size_t)i * kMaxRedBlocks + b] : (i < NS ? 0.0 : -INFINITY);
-        }
+            for (int q = 0; q < 4; ++q) {
+                const int b = b0 + q * kBlock;
+                a[q] = combine<NS>(i, a[q], b < nblocks ? P[b] : id);
+            }
         t[i] = combine<NS>(i, combine<NS>(i, a[0], a[1]), combine<NS>(i, a[2], a[3]));
     }
     block_reduce<NS, NM>(t, lds);
@@ -128,7 +132,6 @@ __global__ void __launch_bounds__(kBlock) grid_fold_kernel(GridRed g, int nblock
         tail(t, scalars);
     }
 }
-static_assert(kMaxRedBlocks <= 4 * kBlock, "the fold reads four partials per thread");
 
 inline int blocks_for(long long n, int per_block = kBlock, int cap = kMaxRedBlocks) {
     long long b = (n + per_block - 1) / per_block;

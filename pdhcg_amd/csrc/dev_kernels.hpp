@@ -371,6 +371,20 @@ constexpr int kRxMaxRank = 256;
 
 inline int rx_chunks(int n) { return (n + kRxWaveCols - 1) / kRxWaveCols; }
 
+// R is read once per pass: non-temporal loads keep it from displacing the
+// reused vectors in the caches.
+template <class V> __device__ __forceinline__ double stream_load(const V *p) {
+    return (double)__builtin_nontemporal_load(p);
+}
+
+// Dense R storage: column tiles of kRxWaveCols columns x k rows, each tile
+// row-major and contiguous, the last tile zero padded.  One wave's share of
+// R x (a tile) is one contiguous block, and R'u reads 64-column row pieces
+// kRxWaveCols elements apart.
+__host__ __device__ __forceinline__ size_t rx_tile_index(int j, long long i, int k) {
+    return ((size_t)(i / kRxWaveCols) * k + j) * kRxWaveCols + (size_t)(i % kRxWaveCols);
+}
+
 // Sums four per-lane values s[0..3] over the wave with a transposed
 // butterfly: after the xor-32 and xor-16 exchanges each lane holds one row's
 // partial, so only 7 exchanges are needed instead of 24.  Returns the total of
@@ -387,16 +401,15 @@ __device__ __forceinline__ double wave_sum4(const double (&s)[4], int lane) {
     return u;
 }
 
-// RU rows of R over this lane's kRxLaneCols columns: all RU * kRxLaneCols
-// loads are issued before the multiply-adds (one register block).
+// RU rows of a tile over this lane's kRxLaneCols columns: all RU *
+// kRxLaneCols loads are issued before the multiply-adds (one register block).
 template <int RU, class V>
-__device__ __forceinline__ void rx_rows(const V *R, size_t n, int j0, long long c0, const double (&xv)[kRxLaneCols],
-                                        double (&s)[RU]) {
+__device__ __forceinline__ void rx_rows(const V *T, int j0, int lane, const double (&xv)[kRxLaneCols], double (&s)[RU]) {
     double r[RU][kRxLaneCols];
 #pragma unroll
     for (int a = 0; a < RU; ++a)
 #pragma unroll
-        for (int q = 0; q < kRxLaneCols; ++q) r[a][q] = (double)R[(size_t)(j0 + a) * n + c0 + q * kWave];
+        for (int q = 0; q < kRxLaneCols; ++q) r[a][q] = stream_load(T + (size_t)(j0 + a) * kRxWaveCols + lane + q * kWave);
 #pragma unroll
     for (int a = 0; a < RU; ++a) {
         double t = 0.0;
@@ -406,11 +419,11 @@ __device__ __forceinline__ void rx_rows(const V *R, size_t n, int j0, long long 
     }
 }
 
-// Pass 1 of u = D R x: each wave owns kRxWaveCols consecutive columns (x for
-// them held in registers), streams the rows of R over that chunk eight rows
-// at a time (32 loads in flight per lane, coalesced) and writes one partial
-// sum per (row, chunk): partials[j * nchunks + chunk].  The last chunk, which
-// may extend past column n, takes a guarded path.
+// Pass 1 of u = D R x: each wave owns one column tile (kRxWaveCols columns,
+// x for them held in registers), streams its rows eight at a time (32 loads
+// in flight per lane, one contiguous block per wave) and writes one partial
+// sum per (row, tile): partials[j * nchunks + chunk].  Padding columns of the
+// last tile are zero in R and take x = 0.
 template <class V, class X, bool Gated>
 __global__ void __launch_bounds__(kBlock) rx_partial_kernel(const V *R, const X *x, int n, int k, double *partials,
                                                              int nchunks, const int *gate) {
@@ -418,57 +431,34 @@ __global__ void __launch_bounds__(kBlock) rx_partial_kernel(const V *R, const X 
     const int lane = threadIdx.x % kWave;
     const int chunk = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (chunk >= nchunks) return;
-    const long long base = (long long)chunk * kRxWaveCols + lane, c0 = base;
-    // wave-uniform: the butterflies below need every lane on the same path
-    const bool tail = (long long)(chunk + 1) * kRxWaveCols > n;
+    const long long base = (long long)chunk * kRxWaveCols + lane;
+    const V *T = R + (size_t)chunk * k * kRxWaveCols;
     double xv[kRxLaneCols];
-    if (tail) {
 #pragma unroll
-        for (int q = 0; q < kRxLaneCols; ++q) {
-            const long long c = base + q * kWave;
-            xv[q] = c < n ? (double)x[c] : 0.0;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < kRxLaneCols; ++q) xv[q] = (double)x[base + q * kWave];
+    for (int q = 0; q < kRxLaneCols; ++q) {
+        const long long c = base + q * kWave;
+        xv[q] = c < n ? (double)x[c] : 0.0;
     }
-    int j = 0;
     auto emit = [&](const double (&s4)[4], int j0) {
         const double tot = wave_sum4(s4, lane);
         const int r = lane >> 4;
         if ((lane & 15) == 0 && j0 + r < k) partials[(size_t)(j0 + r) * nchunks + chunk] = tot;
     };
-    if (!tail) {
-        for (; j + 8 <= k; j += 8) {
-            double s[8];
-            rx_rows<8>(R, (size_t)n, j, c0, xv, s);
-            emit({s[0], s[1], s[2], s[3]}, j);
-            emit({s[4], s[5], s[6], s[7]}, j + 4);
+    int j = 0;
+    for (; j + 8 <= k; j += 8) {
+        double s[8];
+        rx_rows<8>(T, j, lane, xv, s);
+        emit({s[0], s[1], s[2], s[3]}, j);
+        emit({s[4], s[5], s[6], s[7]}, j + 4);
+    }
+    for (; j < k; j += 4) {
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int a = 0; a < 4 && j + a < k; ++a) {
+            double s1[1];
+            rx_rows<1>(T, j + a, lane, xv, s1);
+            s4[a] = s1[0];
         }
-        for (; j < k; j += 4) {
-            double s4[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int a = 0; a < 4 && j + a < k; ++a) {
-                double s1[1];
-                rx_rows<1>(R, (size_t)n, j + a, c0, xv, s1);
-                s4[a] = s1[0];
-            }
-            emit(s4, j);
-        }
-    } else {
-        for (; j < k; j += 4) {
-            double s4[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int a = 0; a < 4 && j + a < k; ++a) {
-                const V *row = R + (size_t)(j + a) * n;
-                double t = 0.0;
-#pragma unroll
-                for (int q = 0; q < kRxLaneCols; ++q) {
-                    const long long c = base + q * kWave;
-                    if (c < n) t += (double)row[c] * xv[q];
-                }
-                s4[a] = t;
-            }
-            emit(s4, j);
-        }
+        emit(s4, j);
     }
 }
 
@@ -520,7 +510,7 @@ __global__ void __launch_bounds__(kBlock) rx_fold_kernel(const double *partials,
     if (threadIdx.x == 0) *counter = 0u;
 }
 
-// qx = R'u (+ qx, or + qdiag .* x)  with R dense row-major, one thread per column.
+// qx = R'u (+ qx, or + qdiag .* x)  with R dense in column tiles, one thread per column.
 // Gated launches (inside the BB loop) are a separate instantiation so that
 // profiles separate them from the always-running ones.
 template <class V, class X, bool Gated>
@@ -530,20 +520,57 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
     if (Gated && *gate == 0) return;
     constexpr int JU = 10;  // rows of R loaded before the multiply-adds
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V *col = R + rx_tile_index(0, i, k);  // row j of this column at col[j * kRxWaveCols]
         double s = 0.0;
         int j = 0;
         for (; j + JU <= k; j += JU) {
             double r[JU];
 #pragma unroll
-            for (int a = 0; a < JU; ++a) r[a] = (double)R[(size_t)(j + a) * n + i];
+            for (int a = 0; a < JU; ++a) r[a] = stream_load(col + (size_t)(j + a) * kRxWaveCols);
 #pragma unroll
             for (int a = 0; a < JU; ++a) s += r[a] * u[j + a];
         }
-        for (; j < k; ++j) s += (double)R[(size_t)j * n + i] * u[j];
+        for (; j < k; ++j) s += stream_load(col + (size_t)j * kRxWaveCols) * u[j];
         if (qdiag) s = s + (double)qdiag[i] * (double)x[i];
         else if (accumulate) s = s + (double)qx[i];
         qx[i] = (X)s;
     }
+}
+
+// BB body on the dense low-rank path: the R'u pass fused with the gradient
+// step of bb_gradient_kernel.  Qx_p = R'u (+ diag(Q) x_p, or + the sparse-Q
+// product already in qx_sparse); g = Qx_p + c - A'y + (x_p - x)/tau;
+// partial of s'(g - g_old); grad = g.  Qx_p itself is not stored.
+template <class V, class X>
+__global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const double *u, int n, int k,
+                                                           const X *qx_sparse, const X *qdiag, const X *xp, const X *x,
+                                                           const X *c, const X *aty, X *grad, const X *dirn,
+                                                           double inv_tau, GridRed g, const DevScalars *ds) {
+    if (ds->bb_active == 0) return;
+    constexpr int JU = 10;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V *col = R + rx_tile_index(0, i, k);
+        double s = 0.0;
+        int j = 0;
+        for (; j + JU <= k; j += JU) {
+            double r[JU];
+#pragma unroll
+            for (int a = 0; a < JU; ++a) r[a] = stream_load(col + (size_t)(j + a) * kRxWaveCols);
+#pragma unroll
+            for (int a = 0; a < JU; ++a) s += r[a] * u[j + a];
+        }
+        for (; j < k; ++j) s += stream_load(col + (size_t)j * kRxWaveCols) * u[j];
+        const double xpi = (double)xp[i];
+        if (qdiag) s = s + (double)qdiag[i] * xpi;
+        else if (qx_sparse) s = s + (double)qx_sparse[i];
+        const double q = (double)(X)s;  // the stored product's rounding, as bb_gradient_kernel sees it
+        const double cur = q + (double)c[i] - (double)aty[i] + inv_tau * (xpi - (double)x[i]);
+        const double dg = (double)(X)(cur - (double)grad[i]);
+        grad[i] = (X)cur;
+        v[0] += (double)dirn[i] * dg;
+    }
+    grid_partials<1, 0>(g, v);
 }
 
 // u = D rx for a CSR-stored R (rx from an SpMV), single block.

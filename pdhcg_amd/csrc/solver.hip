@@ -347,7 +347,7 @@ template <class V, class X> class Solver final : public SolverBase {
     bool q_sparse_ = false;          // Q stored as CSR (has off-diagonal entries)
     X *qsdiag_ = nullptr;            // Q stored as its diagonal
     X *qdiag_ = nullptr;             // DIAG_Q prox diagonal (diag + 1e-12)
-    V *Rd_ = nullptr;                // dense R, k x n row-major
+    V *Rd_ = nullptr;                // dense R in column tiles (rx_tile_index), zero padded
     bool r_dense_ = false;
     int dkind_ = 0;
     double *ddiag_ = nullptr, *ddense_ = nullptr, *u_ = nullptr, *rx_partials_ = nullptr, *rx_sums_ = nullptr;
@@ -481,11 +481,12 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         const long long dense_elems = (long long)k_ * n_;
         r_dense_ = dense_elems <= 4LL * sp.R.nnz() + 4096;
         if (r_dense_) {
-            std::vector<double> Rd((size_t)dense_elems, 0.0);
-            for (int j = 0; j < k_; ++j)
-                for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e) Rd[(size_t)j * n_ + sp.R.ci[e]] += sp.R.v[e];
-            Rd_ = dupload<V>(Rd, Rd.size());
             rx_chunks_ = rx_chunks(n_);
+            std::vector<double> Rd((size_t)rx_chunks_ * kRxWaveCols * k_, 0.0);
+            for (int j = 0; j < k_; ++j)
+                for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e)
+                    Rd[rx_tile_index(j, sp.R.ci[e], k_)] += sp.R.v[e];
+            Rd_ = dupload<V>(Rd, Rd.size());
             rx_blocks_ = std::max(1, (rx_chunks_ + kBlock / kWave - 1) / (kBlock / kWave));
             rx_partials_ = dzeros<double>((size_t)rx_chunks_ * k_);
             rx_sums_ = dzeros<double>((size_t)k_);
@@ -825,14 +826,28 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     }
     fold<1, 0>(nb, 0, TailBBStep{1, pre, inner_limit_, inner_tol_});
     const int *gate = &ds_->bb_active;
+    const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
+    const bool fused_body = r_dense_ && (qtype_ == PDHCG_LOW_RANK_Q || plus);
     const int max_bodies = inner_limit_ - 1;
     int issued = 0, chunk = 2;
     while (issued < max_bodies) {
         const int todo = std::min(chunk, max_bodies - issued);
         for (int c = 0; c < todo; ++c) {
-            q_product(xp_, qx_, gate);
-            hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_, grad_,
-                               dirn_, inv, n_, red(), ds_);
+            if (fused_body) {
+                // Q x_p with the gradient step folded into the R'u pass
+                if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, xp_, EpiStore<X>{qx_}, st_, gate);
+                hipLaunchKernelGGL((rx_partial_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, xp_,
+                                   n_, k_, rx_partials_, rx_chunks_, gate);
+                hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
+                                   rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+                hipLaunchKernelGGL((rtu_grad_kernel<V, X>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_,
+                                   (plus && q_sparse_) ? qx_ : nullptr, (plus && !q_sparse_) ? qsdiag_ : nullptr,
+                                   xp_, x_, c_, aty_, grad_, dirn_, inv, red(), ds_);
+            } else {
+                q_product(xp_, qx_, gate);
+                hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_,
+                                   grad_, dirn_, inv, n_, red(), ds_);
+            }
             fold<1, 0>(nb, 0, TailBBAlpha{pre}, gate);
             if (has_cones_)
                 hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, snap_, xp_, n_, gate);
