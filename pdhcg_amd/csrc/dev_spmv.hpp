@@ -84,6 +84,32 @@ template <class V> struct DevCsr {
     V *v = nullptr;
 };
 
+// Sum of val[e] * x[ci[e]] over e = e, e + step, ... < e1, added in that
+// order.  Four entries are taken per round: their indices and values are
+// loaded first and the four gathers of x issued together, instead of one
+// dependent index-then-gather chain per entry.  (Non-temporal loads for the
+// matrix stream were measured slower here: A'y went from 114 to 153 us.)
+template <class V, class X, class Acc>
+__device__ __forceinline__ Acc row_dot(const V *__restrict__ val, const int *__restrict__ ci, const X *__restrict__ x,
+                                       int e, int e1, int step, Acc acc) {
+    for (; e + 3 * step < e1; e += 4 * step) {
+        int c[4];
+        Acc v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            c[q] = ci[e + q * step];
+            v[q] = (Acc)val[e + q * step];
+        }
+        Acc g[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) g[q] = (Acc)x[c[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc += v[q] * g[q];
+    }
+    for (; e < e1; e += step) acc += (Acc)val[e] * (Acc)x[ci[e]];
+    return acc;
+}
+
 // y-side epilogue contract: `epi(row, sum)` is called once per row by exactly
 // one thread.  `gate` (may be null) skips the whole launch when *gate == 0.
 template <class V, class X, class Acc, class Epi>
@@ -111,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) spmv_binned_kernel(DevCsr<V> M, const 
         if (active) {
             row = order[tab.class_start[c] + slot];
             const int e0 = rp[row], e1 = rp[row + 1];
-            for (int e = e0 + lane; e < e1; e += lanes) acc += (Acc)val[e] * (Acc)x[ci[e]];
+            acc = row_dot<V, X, Acc>(val, ci, x, e0 + lane, e1, lanes, acc);
         }
         if (lanes > 1) acc = group_sum(acc, lanes);
         if (active && lane == 0) epi(row, acc);
@@ -120,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) spmv_binned_kernel(DevCsr<V> M, const 
         const int row = order[tab.class_start[7] + local_block];
         const int e0 = rp[row], e1 = rp[row + 1];
         Acc acc = Acc(0);
-        for (int e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (Acc)val[e] * (Acc)x[ci[e]];
+        acc = row_dot<V, X, Acc>(val, ci, x, e0 + threadIdx.x, e1, kBlock, acc);
         acc = wave_sum(acc);
         if ((threadIdx.x % kWave) == 0) part[threadIdx.x / kWave] = acc;
         __syncthreads();
