@@ -5,12 +5,18 @@
 // packed many to a wave and long rows get a whole wave; rows longer than
 // kLongRow get a full 256-thread workgroup with an LDS reduction.  One launch
 // covers every class: the workgroup index selects the class through a small
-// offset table.  Each row's sum is produced in a fixed order (lane-strided
-// partial sums, then an xor-butterfly), so the result is deterministic.
-// The per-row result is handed to an epilogue functor, which lets the PDHG
-// proximal updates run inside the SpMV without writing the product to HBM.
+// offset table.  The matrix is stored in that slot order (row pointers
+// contiguous per class, plus the original row id of each slot), so a row's
+// extent is read coalesced and independently of its output index, instead
+// of through a dependent row-id lookup.  Each row's sum is produced in a
+// fixed order (lane-strided partial sums, then an xor-butterfly), so the
+// result is deterministic.  The per-row result is handed to an epilogue
+// functor, which lets the PDHG proximal updates run inside the SpMV without
+// writing the product to HBM.
 #pragma once
 
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -19,14 +25,6 @@ namespace pdhcg {
 
 constexpr int kSpmvClasses = 8;      // L = 1,2,4,8,16,32,64, and workgroup rows
 constexpr int kLongRow = 2048;       // rows longer than this get a workgroup
-
-struct SpmvPlan {
-    int rows = 0;
-    int *d_order = nullptr;                       // row ids grouped by class
-    int class_start[kSpmvClasses + 1] = {0};      // offsets into d_order
-    int block_start[kSpmvClasses + 1] = {0};      // first workgroup of each class
-    int total_blocks = 0;
-};
 
 // Lanes per row for a row of `len` entries: about four entries per lane.
 inline int spmv_class_of(int len) {
@@ -39,50 +37,79 @@ inline int spmv_class_of(int len) {
     return c;
 }
 
-// Builds the class lists on the host and uploads them (stable in row order).
-inline SpmvPlan make_spmv_plan(int rows, const std::vector<int> &rp) {
-    SpmvPlan p;
-    p.rows = rows;
+struct SpmvClassTable {
+    int class_start[kSpmvClasses + 1];  // first slot of each class
+    int block_start[kSpmvClasses + 1];  // first workgroup of each class
+};
+
+// A CSR matrix in slot order: slot s holds original row row[s], whose
+// entries are ci/v[rp[s] .. rp[s + 1]).
+template <class V> struct BinnedCsr {
+    int rows = 0, cols = 0, nnz = 0, total_blocks = 0;
+    int *rp = nullptr, *row = nullptr, *ci = nullptr;
+    V *v = nullptr;
+    SpmvClassTable tab = {};
+};
+
+template <class T> T *upload_vec(const std::vector<T> &h) {
+    T *d = nullptr;
+    HIP_OK(hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+    if (!h.empty()) HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// Builds the slot order (stable within a class) from a host CSR
+// (rows x cols) and uploads the matrix in that order.
+template <class V>
+BinnedCsr<V> make_binned(int rows, int cols, const std::vector<int> &rp, const std::vector<int> &ci,
+                         const std::vector<double> &val) {
+    BinnedCsr<V> b;
+    b.rows = rows;
+    b.cols = cols;
     std::vector<int> count(kSpmvClasses, 0), cls((size_t)rows);
     for (int r = 0; r < rows; ++r) {
         cls[r] = spmv_class_of(rp[(size_t)r + 1] - rp[r]);
         count[cls[r]]++;
     }
-    p.class_start[0] = 0;
-    for (int c = 0; c < kSpmvClasses; ++c) p.class_start[c + 1] = p.class_start[c] + count[c];
-    std::vector<int> order((size_t)(rows > 0 ? rows : 1)), cursor(p.class_start, p.class_start + kSpmvClasses);
+    b.tab.class_start[0] = 0;
+    for (int c = 0; c < kSpmvClasses; ++c) b.tab.class_start[c + 1] = b.tab.class_start[c] + count[c];
+    std::vector<int> order((size_t)rows), cursor(b.tab.class_start, b.tab.class_start + kSpmvClasses);
     for (int r = 0; r < rows; ++r) order[cursor[cls[r]]++] = r;
-    p.block_start[0] = 0;
+    b.tab.block_start[0] = 0;
     for (int c = 0; c < kSpmvClasses; ++c) {
         long long blocks;
         if (c == 7) blocks = count[c];
-        else {
-            const long long lanes = 1LL << c;
-            blocks = (count[c] * lanes + kBlock - 1) / kBlock;
-        }
-        p.block_start[c + 1] = p.block_start[c] + (int)blocks;
+        else blocks = ((long long)count[c] * (1LL << c) + kBlock - 1) / kBlock;
+        b.tab.block_start[c + 1] = b.tab.block_start[c] + (int)blocks;
     }
-    p.total_blocks = p.block_start[kSpmvClasses];
-    HIP_OK(hipMalloc(&p.d_order, order.size() * sizeof(int)));
-    HIP_OK(hipMemcpy(p.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
-    return p;
+    b.total_blocks = b.tab.block_start[kSpmvClasses];
+    std::vector<int> srp((size_t)rows + 1, 0);
+    for (int s = 0; s < rows; ++s) srp[(size_t)s + 1] = srp[s] + (rp[(size_t)order[s] + 1] - rp[order[s]]);
+    b.nnz = srp[rows];
+    std::vector<int> sci((size_t)b.nnz);
+    std::vector<V> sv((size_t)b.nnz);
+    for (int s = 0; s < rows; ++s) {
+        const int r = order[s];
+        int o = srp[s];
+        for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e, ++o) {
+            sci[o] = ci[e];
+            sv[o] = (V)val[e];
+        }
+    }
+    b.rp = upload_vec(srp);
+    b.row = upload_vec(order);
+    b.ci = upload_vec(sci);
+    b.v = upload_vec(sv);
+    return b;
 }
 
-inline void free_spmv_plan(SpmvPlan &p) {
-    if (p.d_order) (void)hipFree(p.d_order);
-    p.d_order = nullptr;
+template <class V> void free_binned(BinnedCsr<V> &b) {
+    if (b.rp) (void)hipFree(b.rp);
+    if (b.row) (void)hipFree(b.row);
+    if (b.ci) (void)hipFree(b.ci);
+    if (b.v) (void)hipFree(b.v);
+    b = BinnedCsr<V>();
 }
-
-struct SpmvClassTable {
-    int class_start[kSpmvClasses + 1];
-    int block_start[kSpmvClasses + 1];
-};
-
-template <class V> struct DevCsr {
-    int rows = 0, cols = 0, nnz = 0;
-    int *rp = nullptr, *ci = nullptr;
-    V *v = nullptr;
-};
 
 // Sum of val[e] * x[ci[e]] over e = e, e + step, ... < e1, added in that
 // order.  Four entries are taken per round: their indices and values are
@@ -110,21 +137,15 @@ __device__ __forceinline__ Acc row_dot(const V *__restrict__ val, const int *__r
     return acc;
 }
 
-// y-side epilogue contract: `epi(row, sum)` is called once per row by exactly
-// one thread.  `gate` (may be null) skips the whole launch when *gate == 0.
-template <class V, class X, class Acc, class Epi>
-__global__ void __launch_bounds__(kBlock) spmv_binned_kernel(DevCsr<V> M, const X *__restrict__ x,
-                                                              const int *__restrict__ order, SpmvClassTable tab,
-                                                              Epi epi, const int *gate) {
-    if (gate && *gate == 0) return;
-    const int b = blockIdx.x;
+// Workgroup `b` of a binned product (0 <= b < M.total_blocks): calls
+// out(row, sum) once per row it covers, from exactly one thread.
+template <class V, class X, class Acc, class Out>
+__device__ __forceinline__ void binned_block(const BinnedCsr<V> &M, const X *__restrict__ x, int b, Out out) {
+    const SpmvClassTable &tab = M.tab;
     int c = 0;
 #pragma unroll
     for (int k = 1; k < kSpmvClasses; ++k) c += (b >= tab.block_start[k]) ? 1 : 0;
     const int local_block = b - tab.block_start[c];
-    const int *rp = M.rp;
-    const int *ci = M.ci;
-    const V *val = M.v;
     if (c < 7) {
         const int lanes = 1 << c;
         const int t = local_block * kBlock + threadIdx.x;
@@ -135,42 +156,148 @@ __global__ void __launch_bounds__(kBlock) spmv_binned_kernel(DevCsr<V> M, const 
         int row = 0;
         Acc acc = Acc(0);
         if (active) {
-            row = order[tab.class_start[c] + slot];
-            const int e0 = rp[row], e1 = rp[row + 1];
-            acc = row_dot<V, X, Acc>(val, ci, x, e0 + lane, e1, lanes, acc);
+            const int s = tab.class_start[c] + slot;
+            row = M.row[s];
+            acc = row_dot<V, X, Acc>(M.v, M.ci, x, M.rp[s] + lane, M.rp[s + 1], lanes, acc);
         }
         if (lanes > 1) acc = group_sum(acc, lanes);
-        if (active && lane == 0) epi(row, acc);
+        if (active && lane == 0) out(row, acc);
     } else {
         __shared__ Acc part[kBlock / kWave];
-        const int row = order[tab.class_start[7] + local_block];
-        const int e0 = rp[row], e1 = rp[row + 1];
-        Acc acc = Acc(0);
-        acc = row_dot<V, X, Acc>(val, ci, x, e0 + threadIdx.x, e1, kBlock, acc);
+        const int s = tab.class_start[7] + local_block;
+        const int row = M.row[s];
+        Acc acc = row_dot<V, X, Acc>(M.v, M.ci, x, M.rp[s] + threadIdx.x, M.rp[s + 1], kBlock, Acc(0));
         acc = wave_sum(acc);
         if ((threadIdx.x % kWave) == 0) part[threadIdx.x / kWave] = acc;
         __syncthreads();
         if (threadIdx.x == 0) {
-            Acc s = part[0];
-            for (int w = 1; w < kBlock / kWave; ++w) s += part[w];
-            epi(row, s);
+            Acc sum = part[0];
+            for (int w = 1; w < kBlock / kWave; ++w) sum += part[w];
+            out(row, sum);
         }
     }
 }
 
+// y-side epilogue contract: `epi(row, sum)` is called once per row by exactly
+// one thread.  `gate` (may be null) skips the whole launch when *gate == 0.
 template <class V, class X, class Acc, class Epi>
-void spmv_launch(const DevCsr<V> &M, const SpmvPlan &plan, const X *x, Epi epi, hipStream_t s,
-                 const int *gate = nullptr) {
-    if (plan.total_blocks == 0) return;
-    SpmvClassTable tab;
-    for (int i = 0; i <= kSpmvClasses; ++i) {
-        tab.class_start[i] = plan.class_start[i];
-        tab.block_start[i] = plan.block_start[i];
-    }
-    hipLaunchKernelGGL((spmv_binned_kernel<V, X, Acc, Epi>), dim3(plan.total_blocks), dim3(kBlock), 0, s, M, x,
-                       plan.d_order, tab, epi, gate);
+__global__ void __launch_bounds__(kBlock) spmv_binned_kernel(BinnedCsr<V> M, const X *__restrict__ x, Epi epi,
+                                                              const int *gate) {
+    if (gate && *gate == 0) return;
+    binned_block<V, X, Acc>(M, x, blockIdx.x, epi);
+}
+
+template <class V, class X, class Acc, class Epi>
+void spmv_launch(const BinnedCsr<V> &M, const X *x, Epi epi, hipStream_t s, const int *gate = nullptr) {
+    if (M.total_blocks == 0) return;
+    hipLaunchKernelGGL((spmv_binned_kernel<V, X, Acc, Epi>), dim3(M.total_blocks), dim3(kBlock), 0, s, M, x, epi,
+                       gate);
     HIP_OK(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------
+// Column-panel SpMV for gathers larger than one XCD's L2.
+//
+// The columns are split into kPanels contiguous panels and the matrix into one
+// binned CSR per panel (all rows, global column indices).  Workgroup b is
+// dispatched to XCD b % kXcds, so sending workgroup b to panel
+// (b % kXcds) % kPanels makes each XCD gather from one panel's slice of x,
+// which stays in its L2.  Each panel writes its per-row partial sums; a
+// combine kernel adds a row's kPanels partials in panel order and calls the
+// epilogue.
+// ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
+constexpr int kPanels = 4;
+constexpr int kXcdsPerPanel = kXcds / kPanels;
+static_assert(kXcds % kPanels == 0, "the panels must divide the XCDs evenly");
+
+template <class V, class Acc> struct PanelSet {
+    BinnedCsr<V> M[kPanels];
+    Acc *partial;  // [kPanels][rows]
+    int rows;
+};
+
+template <class Acc> struct PanelStore {
+    Acc *out;
+    __device__ void operator()(int r, Acc s) const { out[r] = s; }
+};
+
+template <class V, class X, class Acc>
+__global__ void __launch_bounds__(kBlock) spmv_panel_kernel(PanelSet<V, Acc> S, const X *__restrict__ x,
+                                                             const int *gate) {
+    if (gate && *gate == 0) return;
+    const int xcd = blockIdx.x % kXcds;
+    const int p = xcd % kPanels;
+    const int b = (blockIdx.x / kXcds) * kXcdsPerPanel + xcd / kPanels;
+    if (b >= S.M[p].total_blocks) return;
+    binned_block<V, X, Acc>(S.M[p], x, b, PanelStore<Acc>{S.partial + (size_t)p * S.rows});
+}
+
+template <class Acc, class Epi>
+__global__ void __launch_bounds__(kBlock) spmv_panel_combine_kernel(const Acc *__restrict__ partial, int rows, Epi epi,
+                                                                     const int *gate) {
+    if (gate && *gate == 0) return;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        Acc s = partial[r];
+#pragma unroll
+        for (int p = 1; p < kPanels; ++p) s += partial[(size_t)p * rows + r];
+        epi(r, s);
+    }
+}
+
+// Host side: the panel matrices of one operator.
+template <class V, class Acc> struct PanelSpmv {
+    bool on = false;
+    PanelSet<V, Acc> set = {};
+    int max_blocks = 0;
+
+    // rp/ci/v: host CSR of the operator (rows x cols).
+    void build(int nrows, int ncols, const std::vector<int> &rp, const std::vector<int> &ci,
+               const std::vector<double> &v) {
+        auto panel_of = [&](int col) { return (int)((long long)col * kPanels / std::max(ncols, 1)); };
+        std::vector<std::vector<int>> prp(kPanels, std::vector<int>((size_t)nrows + 1, 0));
+        for (int r = 0; r < nrows; ++r)
+            for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e) prp[panel_of(ci[e])][(size_t)r + 1]++;
+        for (int p = 0; p < kPanels; ++p)
+            for (int r = 0; r < nrows; ++r) prp[p][(size_t)r + 1] += prp[p][r];
+        for (int p = 0; p < kPanels; ++p) {
+            const size_t nnz = (size_t)prp[p][nrows];
+            std::vector<int> pci(nnz);
+            std::vector<double> pv(nnz);
+            std::vector<int> fill(prp[p].begin(), prp[p].end() - 1);
+            for (int r = 0; r < nrows; ++r)
+                for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e)
+                    if (panel_of(ci[e]) == p) {
+                        pci[fill[r]] = ci[e];
+                        pv[fill[r]++] = v[e];
+                    }
+            set.M[p] = make_binned<V>(nrows, ncols, prp[p], pci, pv);
+            max_blocks = std::max(max_blocks, set.M[p].total_blocks);
+        }
+        HIP_OK(hipMalloc(&set.partial, (size_t)kPanels * std::max(nrows, 1) * sizeof(Acc)));
+        set.rows = nrows;
+        on = true;
+    }
+
+    void release() {
+        for (int p = 0; p < kPanels; ++p) free_binned(set.M[p]);
+        if (set.partial) (void)hipFree(set.partial);
+        set = PanelSet<V, Acc>();
+        max_blocks = 0;
+        on = false;
+    }
+
+    template <class X, class Epi> void launch(const X *x, Epi epi, hipStream_t s, const int *gate) const {
+        if (max_blocks > 0) {
+            const int rounds = (max_blocks + kXcdsPerPanel - 1) / kXcdsPerPanel;
+            hipLaunchKernelGGL((spmv_panel_kernel<V, X, Acc>), dim3(kXcds * rounds), dim3(kBlock), 0, s, set, x, gate);
+        }
+        const int blocks = std::max(1, std::min((set.rows + kBlock - 1) / kBlock, 65535));
+        hipLaunchKernelGGL((spmv_panel_combine_kernel<Acc, Epi>), dim3(blocks), dim3(kBlock), 0, s,
+                           (const Acc *)set.partial, set.rows, epi, gate);
+        HIP_OK(hipGetLastError());
+    }
+};
 
 // Plain epilogues.
 template <class X> struct EpiStore {

@@ -12,6 +12,7 @@
 // reductions, scalars, restart and KKT logic in FP64.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -64,22 +65,8 @@ template <class T> T *dzeros(size_t count) {
     return p;
 }
 
-template <class V> DevCsr<V> upload_csr(const HostCSR &h) {
-    DevCsr<V> d;
-    d.rows = h.rows;
-    d.cols = h.cols;
-    d.nnz = h.nnz();
-    d.rp = dupload<int>(h.rp, h.rp.size());
-    d.ci = dupload<int>(h.ci, h.ci.size());
-    d.v = dupload<V>(h.v, h.v.size());
-    return d;
-}
-
-template <class V> void free_csr(DevCsr<V> &d) {
-    if (d.rp) (void)hipFree(d.rp);
-    if (d.ci) (void)hipFree(d.ci);
-    if (d.v) (void)hipFree(d.v);
-    d = DevCsr<V>();
+template <class V> BinnedCsr<V> upload_csr(const HostCSR &h) {
+    return make_binned<V>(h.rows, h.cols, h.rp, h.ci, h.v);
 }
 
 struct RunStats {
@@ -342,8 +329,18 @@ template <class V, class X> class Solver final : public SolverBase {
     double cbr_ = 1, ovr_ = 1, obj_const_ = 0, obj_norm_ = 0, bound_norm_ = 0, rescale_time_ = 0;
     double setup_seconds_ = 0;
 
-    DevCsr<V> A_, At_, Q_, Rc_, Rtc_;
-    SpmvPlan pA_, pAt_, pQ_, pR_, pRt_;
+    BinnedCsr<V> A_, At_, Q_, Rc_, Rtc_;
+    PanelSpmv<V, Acc> pnA_, pnAt_;  // column-panel forms, built when x is larger than an L2
+
+    // y = A x and A'y through the panel form when it is built, with the epilogue
+    template <class Epi> void apply_A(const X *x, Epi epi, hipStream_t s, const int *gate = nullptr) {
+        if (pnA_.on) pnA_.launch(x, epi, s, gate);
+        else spmv_launch<V, X, Acc>(A_, x, epi, s, gate);
+    }
+    template <class Epi> void apply_At(const X *y, Epi epi, hipStream_t s, const int *gate = nullptr) {
+        if (pnAt_.on) pnAt_.launch(y, epi, s, gate);
+        else spmv_launch<V, X, Acc>(At_, y, epi, s, gate);
+    }
     bool q_sparse_ = false;          // Q stored as CSR (has off-diagonal entries)
     X *qsdiag_ = nullptr;            // Q stored as its diagonal
     X *qdiag_ = nullptr;             // DIAG_Q prox diagonal (diag + 1e-12)
@@ -450,11 +447,22 @@ template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const 
 
 template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     A_ = upload_csr<V>(sp.A);
-    pA_ = make_spmv_plan(m_, sp.A.rp);
+    // panels pay off when the gathered vector is larger than one XCD's 4 MiB
+    // L2 (measured on an 8 MB x: 145 -> 109 us; on a 4 MB y they lose,
+    // 99 -> 123 us); PDHCG_SPMV_PANELS=0 turns them off, =1 forces them
+    const char *pe = std::getenv("PDHCG_SPMV_PANELS");
+    const int panel_mode = pe ? std::atoi(pe) : -1;
+    const size_t kPanelMinBytes = (size_t)4 << 20;
+    auto want_panels = [&](int cols, int nnz, int rows) {
+        if (panel_mode == 0) return false;
+        if (panel_mode == 1) return rows > 0;
+        return (size_t)cols * sizeof(X) > kPanelMinBytes && nnz >= 4 * rows;
+    };
+    if (want_panels(n_, sp.A.nnz(), m_)) pnA_.build(m_, n_, sp.A.rp, sp.A.ci, sp.A.v);
     {
         const HostCSR t = transpose(sp.A);
         At_ = upload_csr<V>(t);
-        pAt_ = make_spmv_plan(n_, t.rp);
+        if (want_panels(m_, t.nnz(), n_)) pnAt_.build(n_, m_, t.rp, t.ci, t.v);
     }
     if (qtype_ == PDHCG_SPARSE_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
         bool diag_only = true;
@@ -472,7 +480,6 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         } else {
             q_sparse_ = true;
             Q_ = upload_csr<V>(sp.Q);
-            pQ_ = make_spmv_plan(n_, sp.Q.rp);
         }
     }
     if (qtype_ == PDHCG_DIAG_Q) qdiag_ = dupload<X>(sp.qdiag, (size_t)n_);
@@ -495,8 +502,6 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
             Rc_ = upload_csr<V>(sp.R);
             HostCSR rt = transpose(sp.R);
             Rtc_ = upload_csr<V>(rt);
-            pR_ = make_spmv_plan(k_, sp.R.rp);
-            pRt_ = make_spmv_plan(n_, rt.rp);
             rxv_ = dzeros<X>((size_t)k_);
         }
         dkind_ = (int)sp.dkind;
@@ -637,16 +642,13 @@ template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &s
 
 template <class V, class X> Solver<V, X>::~Solver() {
     if (st_) (void)hipStreamSynchronize(st_);
-    free_csr(A_);
-    free_csr(At_);
-    free_csr(Q_);
-    free_csr(Rc_);
-    free_csr(Rtc_);
-    free_spmv_plan(pA_);
-    free_spmv_plan(pAt_);
-    free_spmv_plan(pQ_);
-    free_spmv_plan(pR_);
-    free_spmv_plan(pRt_);
+    free_binned(A_);
+    free_binned(At_);
+    free_binned(Q_);
+    free_binned(Rc_);
+    free_binned(Rtc_);
+    pnA_.release();
+    pnAt_.release();
     void *ptrs[] = {qsdiag_, qdiag_, Rd_,     ddiag_,  ddense_,      u_,          rx_partials_, rx_counter_, rx_sums_,
                     rxv_,    lb_,    ub_,     c_,      cl_,          cu_,         g_,           cl_fin_,
                     cu_fin_, lb_fin_, ub_fin_, con_rescale_, var_rescale_, x0_,     x_,           xp_,
@@ -683,8 +685,8 @@ template <class V, class X> double Solver<V, X>::power_sv() {
         const int todo = std::min(chunk, P_.sv_max_iter - it);
         for (int c = 0; c < todo; ++c, ++it) {
             X *ev = buf[it % 2], *ne = buf[(it + 1) % 2];
-            spmv_launch<V, X, Acc>(At_, pAt_, ev, EpiStore<X>{dp}, st_, gate);
-            spmv_launch<V, X, Acc>(A_, pA_, dp, EpiStore<X>{ne}, st_, gate);
+            apply_At(ev, EpiStore<X>{dp}, st_, gate);
+            apply_A(dp, EpiStore<X>{ne}, st_, gate);
             hipLaunchKernelGGL(pi_dot_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, red(), ds_);
             fold<1, 0>(grid(m_), 0, TailPiValue{}, gate);
             hipLaunchKernelGGL(pi_residual_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, ne, ev, m_, P_.sv_tol,
@@ -726,7 +728,7 @@ template <class V, class X> double Solver<V, X>::power_eig(bool minimum, double 
             hipLaunchKernelGGL(pi_norm_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, n_, red(), ds_);
             fold<1, 0>(grid(n_), 0, TailPiNorm{}, gate);
             hipLaunchKernelGGL(scale_dev_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, &ds_->pi_inv, n_, gate);
-            spmv_launch<V, X, Acc>(Q_, pQ_, v, EpiStore<X>{av}, st_, gate);
+            spmv_launch<V, X, Acc>(Q_, v, EpiStore<X>{av}, st_, gate);
             hipLaunchKernelGGL(pi_rayleigh_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, v, av, shift,
                                minimum ? 1 : 0, n_, P_.sv_tol, red(), ds_);
             fold<1, 0>(grid(n_), 0, TailPiRayleigh{P_.sv_tol}, gate);
@@ -754,13 +756,13 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
         hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qdiag_, x, qx, n_, gate);
         return;
     case PDHCG_SPARSE_Q:
-        if (q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, x, EpiStore<X>{qx}, st_, gate);
+        if (q_sparse_) spmv_launch<V, X, Acc>(Q_, x, EpiStore<X>{qx}, st_, gate);
         else hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qsdiag_, x, qx, n_, gate);
         return;
     case PDHCG_LOW_RANK_Q:
     case PDHCG_LOW_RANK_PLUS_SPARSE_Q: {
         const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
-        if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, x, EpiStore<X>{qx}, st_, gate);
+        if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, x, EpiStore<X>{qx}, st_, gate);
         if (r_dense_) {
             const X *qd = (plus && !q_sparse_) ? qsdiag_ : nullptr;
             const int acc = (plus && q_sparse_) ? 1 : 0;
@@ -784,13 +786,13 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
                 timer_.stop(kTimedRtu, t1, st_);
             }
         } else {
-            spmv_launch<V, X, Acc>(Rc_, pR_, x, EpiStore<X>{rxv_}, st_, gate);
+            spmv_launch<V, X, Acc>(Rc_, x, EpiStore<X>{rxv_}, st_, gate);
             hipLaunchKernelGGL(apply_middle_kernel<X>, dim3(1), dim3(kBlock), 0, st_, rxv_, u_, k_, ddiag_, ddense_,
                                dkind_, gate);
             if (plus && !q_sparse_)
                 hipLaunchKernelGGL(mul_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, qsdiag_, x, qx, n_, gate);
-            if (plus) spmv_launch<V, double, double>(Rtc_, pRt_, u_, EpiAccumulate<X>{qx}, st_, gate);
-            else spmv_launch<V, double, double>(Rtc_, pRt_, u_, EpiStore<X>{qx}, st_, gate);
+            if (plus) spmv_launch<V, double, double>(Rtc_, u_, EpiAccumulate<X>{qx}, st_, gate);
+            else spmv_launch<V, double, double>(Rtc_, u_, EpiStore<X>{qx}, st_, gate);
         }
         return;
     }
@@ -835,7 +837,7 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
         for (int c = 0; c < todo; ++c) {
             if (fused_body) {
                 // Q x_p with the gradient step folded into the R'u pass
-                if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, pQ_, xp_, EpiStore<X>{qx_}, st_, gate);
+                if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, xp_, EpiStore<X>{qx_}, st_, gate);
                 hipLaunchKernelGGL((rx_partial_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, xp_,
                                    n_, k_, rx_partials_, rx_chunks_, gate);
                 hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
@@ -891,7 +893,7 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
         EpiPrimalStep<X> epi{x_, xp_, xr_, dslack_, c_, lb_, ub_, x0_, qtype_ == PDHCG_DIAG_Q ? qdiag_ : nullptr,
                              tau, w, rho, cones ? 1 : store, fuse_halpern ? 1 : 0};
         const int t = timer_.start(kTimedPrimalStep, st_);
-        spmv_launch<V, X, Acc>(At_, pAt_, y_, epi, st_);
+        apply_At(y_, epi, st_);
         timer_.stop(kTimedPrimalStep, t, st_);
         if (cones) {
             if (qtype_ == PDHCG_DIAG_Q) cones_.project_diag_q(xp_, var_rescale_, qdiag_, tau, st_);
@@ -901,7 +903,7 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
         }
     } else {
         const int t = timer_.start(kTimedPrimalStep, st_);
-        spmv_launch<V, X, Acc>(At_, pAt_, y_, EpiStore<X>{aty_}, st_);
+        apply_At(y_, EpiStore<X>{aty_}, st_);
         timer_.stop(kTimedPrimalStep, t, st_);
         bb_prox(tau, fuse_halpern, w);
     }
@@ -911,7 +913,7 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
         // SpMV and the update (reference src/pdhg_core_op.cu:809-829)
         EpiDualPrepare<X> prep{ax_, aproj_, y_, g_, cl_, cu_, sigma};
         const int t = timer_.start(kTimedDualStep, st_);
-        spmv_launch<V, X, Acc>(A_, pA_, xr_, prep, st_);
+        apply_A(xr_, prep, st_);
         timer_.stop(kTimedDualStep, t, st_);
         acones_.project(aproj_, aff_d_, false, nullptr, st_);
         hipLaunchKernelGGL(affine_dual_finish_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, y_, yp_, yr_, ax_,
@@ -920,7 +922,7 @@ template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
     }
     EpiDualStep<X> depi{y_, yp_, yr_, g_, cl_, cu_, y0_, sigma, w, rho, store, fuse_halpern ? 1 : 0};
     const int t = timer_.start(kTimedDualStep, st_);
-    spmv_launch<V, X, Acc>(A_, pA_, xr_, depi, st_);
+    apply_A(xr_, depi, st_);
     timer_.stop(kTimedDualStep, t, st_);
 }
 
@@ -936,7 +938,7 @@ template <class V, class X> void Solver<V, X>::fixed_point_error() {
     fold<2, 0>(grid((long long)n_ + m_));
     read_scalars();
     const double pn2 = hs_->red[0], dn2 = hs_->red[1];
-    spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
+    apply_At(dy_, EpiStore<X>{aty_}, st_);
     hipLaunchKernelGGL(dot_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, aty_, dx_, n_, red(), nullptr);
     fold<1, 0>(grid(n_));
     read_scalars();
@@ -1000,8 +1002,8 @@ template <class V, class X> bool Solver<V, X>::should_restart() {
 template <class V, class X> void Solver<V, X>::residual() {
     const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
     const double tau = step_ / pw_;
-    spmv_launch<V, X, Acc>(A_, pA_, xp_, EpiStore<X>{ax_}, st_);
-    spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
+    apply_A(xp_, EpiStore<X>{ax_}, st_);
+    apply_At(yp_, EpiStore<X>{aty_}, st_);
     q_product(xp_, qx_, nullptr);
     const GridRed r0 = red();
     if (has_affine_) {
@@ -1093,8 +1095,8 @@ template <class V, class X> void Solver<V, X>::infeasibility() {
     read_scalars();
     const double prn = hs_->red[0], drn = hs_->red[1];
     if (prn > 0.0) hipLaunchKernelGGL(scale_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, dx_, 1.0 / prn, n_);
-    spmv_launch<V, X, Acc>(A_, pA_, dx_, EpiStore<X>{ax_}, st_);
-    spmv_launch<V, X, Acc>(At_, pAt_, dy_, EpiStore<X>{aty_}, st_);
+    apply_A(dx_, EpiStore<X>{ax_}, st_);
+    apply_At(dy_, EpiStore<X>{aty_}, st_);
     const GridRed r0 = red();
     hipLaunchKernelGGL(ray_rows_kernel<X>, dim3(grid(m_)), dim3(kBlock), 0, st_, dy_, ax_, g_, cl_, cu_, cl_fin_,
                        cu_fin_, con_rescale_, m_, r0);
@@ -1212,7 +1214,7 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
 
 // reference src/pdhg_core_op.cu:1431-1496
 template <class V, class X> pdhcg_result_t *Solver<V, X>::result(const qp_problem_t *orig) {
-    spmv_launch<V, X, Acc>(At_, pAt_, yp_, EpiStore<X>{aty_}, st_);
+    apply_At(yp_, EpiStore<X>{aty_}, st_);
     q_product(xp_, qx_, nullptr);
     double *xo = dalloc<double>(n_), *yo = dalloc<double>(m_), *rco = dalloc<double>(n_);
     hipLaunchKernelGGL(result_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, yp_, c_,

@@ -78,6 +78,16 @@ def test_lowrank_column_chunking(n):
     _check_pair(p, 1e-6)
 
 
+@pytest.mark.parametrize("kind", ["lp", "lowrank_diag", "sparse"])
+def test_column_panel_spmv_matches_oracle(kind, monkeypatch):
+    # the column-panel SpMV (used when x is larger than an XCD's L2) forced
+    # on small problems, including A' with fewer columns (m = 3) than panels
+    monkeypatch.setenv("PDHCG_SPMV_PANELS", "1")
+    for n, m in ((300, 120), (6, 4), (5, 3)):
+        p = workloads.random_problem(kind, seed=n + m, n=n, m=m)
+        _check_pair(p, 1e-6)
+
+
 def test_sparse_lowrank_csr_path():
     p = workloads.random_problem("lowrank_sparse_r", seed=3, n=1000, m=150)
     _check_pair(p, 1e-4)
