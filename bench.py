@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+TIMING_PERIOD = 8      # event-time every 8th launch of each instrumented kernel
 METRIC = "time-to-1e-6 KKT & PDHG iter/s: plain vs mixed/batched/warm; % HBM roofline"
 
 
@@ -72,7 +73,7 @@ def run_gpu(p, precision, steps, warmup, eps, max_solve_s):
     t_solve0 = time.perf_counter()
     rc = st.run(warmup)
     st.synchronize()
-    L.pdhcg_solver_timing(st.s, 1)
+    L.pdhcg_solver_timing(st.s, TIMING_PERIOD)
     _, inner0 = st.counts()
     t0 = time.perf_counter()
     rc = st.run(steps)
@@ -193,15 +194,17 @@ def main():
 
     kb = kernel_bytes(p, args.precision)
     kt = main_run["kernel_times"]
-    # dominant kernel = largest GPU time per step.  The R passes run once per
-    # Q product, but only their launches outside the BB loop are event-timed
-    # (the loop's launches are gated and include no-ops); their per-step time
-    # is the timed average times the Q-product count of the window.
+    # dominant kernel = largest GPU time per step.  Every TIMING_PERIOD-th
+    # launch of each instrumented kernel is event-timed.  The SpMVs run once
+    # per step.  The R passes run once per Q product, but only their launches
+    # outside the BB loop are event-timed (the loop's launches are gated and
+    # include no-ops); their per-step time is the timed average times the
+    # Q-product count of the window.
     def per_step_ms(k):
         ms, cnt = kt[k]
         if cnt == 0:
             return 0.0
-        runs = main_run["q_products"] if k in (0, 1) else cnt
+        runs = main_run["q_products"] if k in (0, 1) else args.steps
         return ms / cnt * runs / args.steps
     dom = max(kt, key=per_step_ms)
     names = {0: "rx_partial_kernel (R x partial sums)", 1: "rtu_dense_kernel (R'u + diag Q)",

@@ -45,9 +45,12 @@ pdhcg_result_t *pdhcg_solver_result(pdhcg_solver_t *solver);
 void pdhcg_solver_free(pdhcg_solver_t *solver);
 
 /* Kernel timing with HIP events recorded on the solver's stream.
- * pdhcg_solver_timing(s, 1) clears and enables recording; afterwards
- * pdhcg_solver_kernel_time returns the summed duration and launch count of
- * one of the instrumented kernels:
+ * pdhcg_solver_timing(s, p) with p >= 1 clears and enables recording of every
+ * p-th launch of each instrumented kernel (an event pair leaves the GPU idle
+ * for a few microseconds, so p > 1 keeps the overhead off the iteration);
+ * p = 0 disables it.  pdhcg_solver_kernel_time then returns the summed
+ * duration and the count of the recorded launches of one of the
+ * instrumented kernels:
  *   0  R x per-chunk partial sums (dense low-rank R), ungated launches
  *   1  R'u plus the diagonal of Q (dense low-rank R), ungated launches
  *   2  A'y SpMV of the PDHG step (fused primal epilogue, or stored for BB)

@@ -1,6 +1,8 @@
 // Optional per-kernel timing with HIP events recorded on the solver's stream
 // around selected launches; used by the benchmark to turn algorithmic bytes
-// into achieved bandwidth for the dominant kernels.
+// into achieved bandwidth for the dominant kernels.  An event pair costs about
+// 5 us of idle GPU time between the kernels it brackets, so a sampling period
+// records only every period-th launch of each kernel.
 #pragma once
 
 #include <vector>
@@ -20,12 +22,15 @@ enum TimedKernel {
 class KernelTimer {
   public:
     bool enabled = false;
+    int period = 1;  // record every period-th launch of each kernel
+
 
     ~KernelTimer() { reset(true); }
 
     // Returns a started pair slot; call stop() with the returned index.
     int start(int id, hipStream_t s) {
         if (!enabled) return -1;
+        if (calls_[id]++ % period != 0) return -1;
         hipEvent_t a = take(), b = take();
         HIP_OK(hipEventRecord(a, s));
         rec_[id].push_back({a, b});
@@ -53,6 +58,7 @@ class KernelTimer {
                 pool_.push_back(p.second);
             }
         for (auto &r : rec_) r.clear();
+        for (int &c : calls_) c = 0;
         if (destroy) {
             for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
             pool_.clear();
@@ -72,6 +78,7 @@ class KernelTimer {
     }
     std::vector<hipEvent_t> pool_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> rec_[kNumTimed];
+    int calls_[kNumTimed] = {0, 0, 0, 0};
 };
 
 }  // namespace pdhcg

@@ -382,6 +382,9 @@ template <class V, class X> class Solver final : public SolverBase {
     // reductions and device scalars
     double *red_partials_ = nullptr;
     DevScalars *ds_ = nullptr, *hs_ = nullptr;
+    int *bb_poll_ = nullptr;   // pinned: two (bb_active, bb_iter) poll slots
+    hipEvent_t bb_ev_[2] = {nullptr, nullptr};
+    int bb_pred_ = 2;          // BB bodies the previous inner solve ran
 
     // outer-loop state (reference internal/internal_types.h:137-236)
     double q_norm_ = 0, nonconvexity_ = 0;
@@ -415,6 +418,8 @@ template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const 
     ds_ = dzeros<DevScalars>(1);
     HIP_OK(hipHostMalloc((void **)&hs_, sizeof(DevScalars), hipHostMallocDefault));
     std::memset(hs_, 0, sizeof(DevScalars));
+    HIP_OK(hipHostMalloc((void **)&bb_poll_, 4 * sizeof(int), hipHostMallocDefault));
+    for (hipEvent_t &e : bb_ev_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     upload(sp);
 
     // quadratic term information (reference src/solver_state.cu:380-420)
@@ -660,6 +665,9 @@ template <class V, class X> Solver<V, X>::~Solver() {
     cones_.release();
     acones_.release();
     if (hs_) (void)hipHostFree(hs_);
+    if (bb_poll_) (void)hipHostFree(bb_poll_);
+    for (hipEvent_t e : bb_ev_)
+        if (e) (void)hipEventDestroy(e);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -831,44 +839,65 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
     const bool fused_body = r_dense_ && (qtype_ == PDHCG_LOW_RANK_Q || plus);
     const int max_bodies = inner_limit_ - 1;
-    int issued = 0, chunk = 2;
-    while (issued < max_bodies) {
-        const int todo = std::min(chunk, max_bodies - issued);
-        for (int c = 0; c < todo; ++c) {
-            if (fused_body) {
-                // Q x_p with the gradient step folded into the R'u pass
-                if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, xp_, EpiStore<X>{qx_}, st_, gate);
-                hipLaunchKernelGGL((rx_partial_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, xp_,
-                                   n_, k_, rx_partials_, rx_chunks_, gate);
-                hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
-                                   rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
-                hipLaunchKernelGGL((rtu_grad_kernel<V, X>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_,
-                                   (plus && q_sparse_) ? qx_ : nullptr, (plus && !q_sparse_) ? qsdiag_ : nullptr,
-                                   xp_, x_, c_, aty_, grad_, dirn_, inv, red(), ds_);
-            } else {
-                q_product(xp_, qx_, gate);
-                hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_,
-                                   grad_, dirn_, inv, n_, red(), ds_);
-            }
-            fold<1, 0>(nb, 0, TailBBAlpha{pre}, gate);
-            if (has_cones_)
-                hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, snap_, xp_, n_, gate);
-            hipLaunchKernelGGL(bb_direction_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, grad_, dirn_, lb_, ub_,
-                               mi, md, n_, red(), ds_, defer);
-            if (has_cones_) {
-                project_cones(xp_, false, gate);
-                hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, snap_, dirn_, md, n_,
-                                   0, red(), ds_);
-            }
-            fold<1, 0>(nb, 0, TailBBStep{0, pre, inner_limit_, inner_tol_}, gate);
+    auto enqueue_body = [&]() {
+        if (fused_body) {
+            // Q x_p with the gradient step folded into the R'u pass
+            if (plus && q_sparse_) spmv_launch<V, X, Acc>(Q_, xp_, EpiStore<X>{qx_}, st_, gate);
+            hipLaunchKernelGGL((rx_partial_kernel<V, X, true>), dim3(rx_blocks_), dim3(kBlock), 0, st_, Rd_, xp_,
+                               n_, k_, rx_partials_, rx_chunks_, gate);
+            hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
+                               rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
+            hipLaunchKernelGGL((rtu_grad_kernel<V, X>), dim3(nb), dim3(kBlock), 0, st_, Rd_, u_, n_, k_,
+                               (plus && q_sparse_) ? qx_ : nullptr, (plus && !q_sparse_) ? qsdiag_ : nullptr,
+                               xp_, x_, c_, aty_, grad_, dirn_, inv, red(), ds_);
+        } else {
+            q_product(xp_, qx_, gate);
+            hipLaunchKernelGGL(bb_gradient_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, x_, c_, aty_, qx_,
+                               grad_, dirn_, inv, n_, red(), ds_);
         }
-        issued += todo;
+        fold<1, 0>(nb, 0, TailBBAlpha{pre}, gate);
+        if (has_cones_)
+            hipLaunchKernelGGL(copy_gated_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, snap_, xp_, n_, gate);
+        hipLaunchKernelGGL(bb_direction_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, grad_, dirn_, lb_, ub_,
+                           mi, md, n_, red(), ds_, defer);
+        if (has_cones_) {
+            project_cones(xp_, false, gate);
+            hipLaunchKernelGGL(bb_cone_step_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, xp_, snap_, dirn_, md, n_,
+                               0, red(), ds_);
+        }
+        fold<1, 0>(nb, 0, TailBBStep{0, pre, inner_limit_, inner_tol_}, gate);
+    };
+    // The loop stops on the device (bb_active); the host learns of it through
+    // polls of (bb_active, bb_iter) copied after a chunk of bodies.  One more
+    // chunk is always queued behind the polled one, so the stream never
+    // drains while the host waits: when the polled flag says the loop ended,
+    // the queued chunk runs gated (empty launches).  The first chunk is the
+    // body count of the previous solve.
+    int issued = 0, inflight = 0, head = 0, chunk = std::max(1, bb_pred_);
+    auto enqueue_chunk = [&](int count) {
+        count = std::min(count, max_bodies - issued);
+        for (int c = 0; c < count; ++c) enqueue_body();
+        issued += count;
         HIP_OK(hipGetLastError());
-        if (issued >= max_bodies) break;
-        HIP_OK(hipMemcpyAsync(&hs_->bb_active, &ds_->bb_active, sizeof(int), hipMemcpyDeviceToHost, st_));
-        HIP_OK(hipStreamSynchronize(st_));
-        if (!hs_->bb_active) break;
-        chunk = std::min(chunk * 2, 16);
+        const int slot = (head + inflight) % 2;
+        HIP_OK(hipMemcpyAsync(bb_poll_ + 2 * slot, &ds_->bb_active, 2 * sizeof(int), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipEventRecord(bb_ev_[slot], st_));
+        ++inflight;
+    };
+    while (issued < max_bodies || inflight > 0) {
+        while (issued < max_bodies && inflight < 2) {
+            enqueue_chunk(chunk);
+            chunk = inflight == 1 ? 1 : std::min(chunk * 2, 16);
+        }
+        HIP_OK(hipEventSynchronize(bb_ev_[head]));
+        const int active = bb_poll_[2 * head], iter = bb_poll_[2 * head + 1];
+        head ^= 1;
+        --inflight;
+        if (!active) {
+            bb_pred_ = std::max(1, iter - 1);
+            break;
+        }
+        if (inflight == 0) chunk = std::max(chunk, 2);
     }
     hipLaunchKernelGGL(bb_final_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, x_, xp_, xr_, x0_, n_,
                        fuse_halpern ? 1 : 0, w, P_.reflection_coefficient, ds_);
@@ -1388,8 +1417,10 @@ int pdhcg_solver_timing(pdhcg_solver_t *s, int enable) {
     if (!s) return -1;
     try {
         s->impl->synchronize();
+        if (enable < 0) throw std::runtime_error("timing period must be >= 0");
         s->impl->timer().reset();
         s->impl->timer().enabled = enable != 0;
+        s->impl->timer().period = enable > 0 ? enable : 1;
         return 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[interface] %s\n", e.what());

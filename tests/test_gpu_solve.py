@@ -6,6 +6,8 @@ Bar (FP64 form): same termination status, objective within
 independent numpy KKT check at the solve tolerance (x 10 slack for the
 max-norm box term, which the solvers measure in the scaled space).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -106,3 +108,28 @@ def test_iteration_limit_and_counts():
     o = oracle_solve(p, eps=1e-12, iteration_limit=57)
     assert g.status == o.status == "ITERATION_LIMIT"
     assert g.iterations == o.iterations == 57
+
+
+def test_stepper_sampled_kernel_timing():
+    # pdhcg_solver_timing(s, p) event-times every p-th launch of the
+    # instrumented kernels; the A'y and A x_bar SpMVs run once per step
+    p = workloads.random_problem("lowrank_diag", seed=4, n=600, m=150)
+    st = pdhcg_amd.Stepper(p, eps=1e-12)
+    try:
+        st.run(5)
+        st.synchronize()
+        L = st.L
+        assert L.pdhcg_solver_timing(st.s, -1) != 0
+        assert L.pdhcg_solver_timing(st.s, 3) == 0
+        t0, _ = st.counts()
+        assert st.run(12) == 0
+        st.synchronize()
+        t1, _ = st.counts()
+        assert t1 - t0 == 12
+        for kid in (2, 3):
+            ms, cnt = C.c_double(), C.c_int()
+            assert L.pdhcg_solver_kernel_time(st.s, kid, C.byref(ms), C.byref(cnt)) == 0
+            assert cnt.value == 4 and ms.value > 0.0, (kid, cnt.value, ms.value)
+        assert L.pdhcg_solver_timing(st.s, 0) == 0
+    finally:
+        st.close()
