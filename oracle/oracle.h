@@ -85,6 +85,17 @@ void orc_solver_scalings(const orc_solver_t *s, double *con_rescale, double *var
 int orc_project_block(int type, double *x, int v_dim, const double *d, double alpha);
 /* Dual-cone residual (rc - P_{K*}(rc)) scaled by d, written to out. */
 int orc_dual_residual_block(int type, const double *rc, int v_dim, const double *d, double alpha, double *out);
+/* Fixed sections (orc_section.c): the block's fixed slots (fixed[i] != 0)
+ * keep their value and the free ones are projected onto the section of D K in
+ * the metric diag(metric d^2) (metric NULL: 1).  SOC, RSOC, EXP and POWER;
+ * returns -1 for other types. */
+int orc_section_project(int type, double *x, int v_dim, const double *d, const double *metric, const char *fixed,
+                        double alpha);
+/* Residual of a block with fixed slots: (x - P(x - rc)) d on the free slots,
+ * 0 on the fixed ones, P the section projection in the metric d^2. */
+void orc_section_dual_residual(int type, const double *x, const double *rc, int v_dim, const double *d,
+                               const char *fixed, double alpha, double *out);
+int orc_section_has_fixed(const char *fixed, int len);
 
 /* Permutations and partition cuts (orc_permute.c; reference src/permute.cu,
  * src/partition_utils.c, distributed/distributed_utils.cu:179-216).  method:

@@ -16,7 +16,7 @@
  * dense, eigen-decomposition, sum of max(lambda, 0) u u', dense -> svec.  The
  * eigen-decomposition here is a cyclic Jacobi (Golub & Van Loan, sec. 8.5)
  * run to full FP64 convergence; the reference calls cuSOLVER syevj/syevd.
- * Fixed cone sections are not restated.
+ * Fixed cone sections: orc_section.c.
  */
 #include "orc_cones.h"
 #include "oracle.h"

@@ -78,9 +78,13 @@ static void project_one(cone_type_t type, double *v, int vd, const double *d, do
 /* Applies every block's projection onto diag(d) K to v (reference
  * src/cone_dispatch.cu:38-57). */
 static void project_blocks(const cone_blocks_t *b, const double *d, double *v, double *warm) {
-    for (int k = 0; k < b->num_cones; ++k)
-        project_one(b->type[k], v + b->start_idx[k], b->v_dim[k], d + b->start_idx[k], warm + k,
-                    b->power_alpha ? b->power_alpha[k] : 0.0);
+    for (int k = 0; k < b->num_cones; ++k) {
+        const int st = b->start_idx[k];
+        const double al = b->power_alpha ? b->power_alpha[k] : 0.0;
+        if (b->is_fixed && orc_section_has_fixed(b->is_fixed + st, orc_cone_block_length(b, k)))
+            orc_section_project(b->type[k], v + st, b->v_dim[k], d + st, NULL, b->is_fixed + st, al);
+        else project_one(b->type[k], v + st, b->v_dim[k], d + st, warm + k, al);
+    }
 }
 
 /* Variable cone blocks: projection onto diag(var_rescale) K. */
@@ -98,6 +102,19 @@ static void project_cones_diag_q(orc_solver_t *s, double *v, double tau) {
     const double *d = s->w->var_rescale, *q = s->w->qdiag;
     for (int k = 0; k < b->num_cones; ++k) {
         int st = b->start_idx[k], L = orc_cone_block_length(b, k);
+        if (b->is_fixed && orc_section_has_fixed(b->is_fixed + st, L)) {
+            /* fixed section in the metric diag(w) directly (reference
+               src/kernels/pdhcg_soc_cone_kernels.cu:1463-1467 and siblings) */
+            double *mw = (double *)malloc((size_t)L * sizeof(double));
+            for (int i = 0; i < L; ++i) {
+                double wi = 1.0 + tau * q[st + i];
+                mw[i] = wi > 0.0 ? wi : 1.0;
+            }
+            orc_section_project(b->type[k], v + st, b->v_dim[k], d + st, mw, b->is_fixed + st,
+                                b->power_alpha ? b->power_alpha[k] : 0.0);
+            free(mw);
+            continue;
+        }
         double *sw = (double *)malloc((size_t)L * sizeof(double)), *e = (double *)malloc((size_t)L * sizeof(double));
         for (int i = 0; i < L; ++i) {
             double wi = 1.0 + tau * q[st + i];
@@ -374,10 +391,8 @@ static void init_inner(orc_solver_t *s) {
 
 orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_t *params) {
     if (!prob) return NULL;
-    int any_fixed = 0;
-    for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i) any_fixed |= prob->cones.is_fixed[i] != 0;
-    if (prob->num_quadratic_constraints > 0 || any_fixed) {
-        fprintf(stderr, "[oracle] fixed cone sections and quadratic constraints are not restated.\n");
+    if (prob->num_quadratic_constraints > 0) {
+        fprintf(stderr, "[oracle] quadratic constraints are not restated.\n");
         return NULL;
     }
     orc_solver_t *s = (orc_solver_t *)calloc(1, sizeof(orc_solver_t));
@@ -465,16 +480,22 @@ orc_solver_t *orc_solver_create(const qp_problem_t *prob, const pdhg_parameters_
     s->res_warm = zeros(work->cones.num_cones);
     s->pgp = zeros(n);
     if (s->has_cones && w->qtype != PDHCG_NON_Q) s->eff = zeros(n);
+    const char *pin = work->cones.is_fixed;
     for (int k = 0; k < work->cones.num_cones; ++k) {
         int aux = work->cones.start_idx[k] + work->cones.v_dim[k];
         double *arrs[4] = {s->x0, s->x, s->xp, s->xr};
+        /* fixed slots keep their start value (reference :650-651,673) */
         if (work->cones.type[k] == CONE_STANDARD_SOC) {
             double wv = -0.7071067811865475 * w->cbr * w->var_rescale[aux];
             double zv = 0.7071067811865475 * w->cbr * w->var_rescale[aux + 1];
-            for (int a = 0; a < 4; ++a) { arrs[a][aux] = wv; arrs[a][aux + 1] = zv; }
+            for (int a = 0; a < 4; ++a) {
+                if (!(pin && pin[aux])) arrs[a][aux] = wv;
+                if (!(pin && pin[aux + 1])) arrs[a][aux + 1] = zv;
+            }
         } else if (work->cones.type[k] == CONE_ROTATED_SOC) {
             double tv = w->cbr * w->var_rescale[aux + 1];
-            for (int a = 0; a < 4; ++a) arrs[a][aux + 1] = tv;
+            for (int a = 0; a < 4; ++a)
+                if (!(pin && pin[aux + 1])) arrs[a][aux + 1] = tv;
         }
     }
     if (s->has_cones) project_cones(s, s->x0, s->proj_warm);
@@ -763,6 +784,13 @@ static void residual(orc_solver_t *s) {
             int st = b->start_idx[k], vd = b->v_dim[k], L = orc_cone_block_length(b, k);
             for (int i = st; i < st + L; ++i) rc[i] = eff[i] - s->aty[i];
             const double *d = w->var_rescale + st;
+            if (b->is_fixed && orc_section_has_fixed(b->is_fixed + st, L)) {
+                /* prox residual of the section (reference
+                   src/kernels/pdhcg_soc_cone_kernels.cu:489-507) */
+                orc_section_dual_residual(b->type[k], s->xp + st, rc + st, vd, d, b->is_fixed + st,
+                                          b->power_alpha ? b->power_alpha[k] : 0.0, s->dres + st);
+                continue;
+            }
             switch (b->type[k]) {
             case CONE_STANDARD_SOC: orc_soc_dual_residual(rc + st, vd, d, s->res_warm + k, s->dres + st); break;
             case CONE_ROTATED_SOC: orc_rsoc_dual_residual(rc + st, vd, d, s->res_warm + k, s->dres + st); break;

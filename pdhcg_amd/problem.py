@@ -35,6 +35,7 @@ class QPData:
     fmt: dict = field(default_factory=dict)  # per-matrix input format, default 'csr'
     primal_start: Optional[np.ndarray] = None
     dual_start: Optional[np.ndarray] = None
+    cone_fixed: Sequence[tuple] = ()  # (cone index, slot, value) applied with set_cone_fixed
 
     @property
     def n(self) -> int:
@@ -100,6 +101,10 @@ def build(lib, prefix: str, p: QPData) -> Optional[Handle]:
     if not ptr:
         return None
     h = Handle(lib, prefix, ptr, [])
+    for cone, slot, value in p.cone_fixed:
+        if getattr(lib, prefix + "set_cone_fixed")(ptr, int(cone), int(slot), float(value)) != 0:
+            h.close()
+            return None
     if p.primal_start is not None or p.dual_start is not None:
         ps, ds = arr(p.primal_start), arr(p.dual_start)
         getattr(lib, prefix + "set_start_values")(ptr, capi.as_ptr(ps), capi.as_ptr(ds))
