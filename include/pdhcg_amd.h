@@ -64,7 +64,13 @@ int pdhcg_solver_kernel_time(pdhcg_solver_t *solver, int kernel, double *total_m
  * d is the per-slot scaling (NULL: all ones); the projection is onto
  * diag(d) K for every block of `cones`, in place on x.  The dual residual
  * writes (rc - P_{K*}(rc)) * d on the cone slots of out and leaves the other
- * slots of out as they are.  Return 0, or -1 with a message on stderr. */
+ * slots of out as they are.  Blocks with fixed slots (cones->is_fixed, of at
+ * least n entries) are projected onto their fixed section: the fixed slots
+ * keep their value and the free ones move to the nearest point, in the metric
+ * d^2, that puts the block in the cone (reference
+ * src/kernels/pdhcg_soc_cone_kernels.cu:23-260 and its siblings); the dual
+ * residual rejects such blocks (their residual needs a primal point).
+ * Return 0, or -1 with a message on stderr. */
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n);
 int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n);
 

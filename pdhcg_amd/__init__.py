@@ -70,13 +70,15 @@ def solve(p: QPData, precision: int = PRECISION_FP64, **params) -> Optional[capi
     return _solve(L, "", p, default_params(L, "", **params))
 
 
-def project_cones(specs, x, d=None) -> np.ndarray:
+def project_cones(specs, x, d=None, fixed=None) -> np.ndarray:
     """Device projection of x onto diag(d) K for the blocks in ``specs``
-    ((type, start, v_dim[, alpha]) tuples); returns a new array."""
+    ((type, start, v_dim[, alpha]) tuples); returns a new array.  ``fixed``
+    (per-variable 0/1) marks slots that keep their value: those blocks are
+    projected onto their fixed section."""
     L = lib()
     out = capi.carray(x).copy()
     dd = None if d is None else capi.carray(d)
-    b, _keep = capi.cone_blocks(specs)
+    b, _keep = capi.cone_blocks(specs, fixed)
     if L.pdhcg_project_cones(C.byref(b), capi.as_ptr(dd), capi.as_ptr(out), len(out)) != 0:
         raise RuntimeError("pdhcg_project_cones failed")
     return out

@@ -283,8 +283,9 @@ def cone_specs(specs) -> Any:
     return arr
 
 
-def cone_blocks(specs):
-    """Builds a cone_blocks_t from (type, start, v_dim[, alpha]) tuples.
+def cone_blocks(specs, fixed=None):
+    """Builds a cone_blocks_t from (type, start, v_dim[, alpha]) tuples;
+    fixed, if given, is the per-variable 0/1 fixed-slot mask.
 
     Returns (struct, keep) where keep holds the arrays the struct points to."""
     k = len(specs)
@@ -300,7 +301,12 @@ def cone_blocks(specs):
     b.power_alpha = C.cast(alpha, c_double_p)
     b.fixed_mask_size = 0
     b.is_fixed = None
-    return b, (start, vdim, typ, alpha)
+    mask = None
+    if fixed is not None:
+        mask = C.create_string_buffer(bytes([1 if v else 0 for v in fixed]), len(fixed))
+        b.fixed_mask_size = len(fixed)
+        b.is_fixed = C.cast(mask, C.c_char_p)
+    return b, (start, vdim, typ, alpha, mask)
 
 
 @dataclass

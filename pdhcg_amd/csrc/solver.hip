@@ -21,6 +21,7 @@
 
 #include "../../include/pdhcg_amd.h"
 #include "dev_cones.hpp"
+#include "dev_sections.hpp"
 #include "dev_kernels.hpp"
 #include "dev_timer.hpp"
 #include "host_scale.h"
@@ -87,6 +88,9 @@ class DevConeSet {
   public:
     bool psd = false, power = false;
     ConeList rsoc, soc, expc, powc;
+    ConeList powall;                             // every power block, for the primal violation
+    SectionList sec;                             // blocks with fixed slots (any non-PSD type)
+    char *fixed = nullptr;                       // per-slot fixed flags, when sec.count > 0
     std::vector<PsdGroup> groups;
     char *mask = nullptr;
     int total = 0;                               // all blocks, in input order:
@@ -105,6 +109,10 @@ class DevConeSet {
         std::vector<char> hmask((size_t)n, 0);
         std::vector<std::pair<int, int>> ps;  // (order, start)
         std::vector<int> hs, hl;
+        if (b->is_fixed && b->fixed_mask_size < n)
+            fail("the fixed-slot mask has %d entries; %d variables", b->fixed_mask_size, n);
+        std::vector<int> sec_start, sec_vdim, sec_type, pall_start;
+        std::vector<double> sec_alpha, pall_alpha;
         for (int k = 0; k < b->num_cones; ++k) {
             const int st = b->start_idx[k], vd = b->v_dim[k], len = cone_block_slots(b, k);
             hs.push_back(st);
@@ -120,7 +128,20 @@ class DevConeSet {
                 ps.emplace_back(vd, st);
                 continue;
             }
-            if (t == CONE_POWER) power = true;
+            if (t == CONE_POWER) {
+                power = true;
+                pall_start.push_back(st);
+                pall_alpha.push_back(b->power_alpha ? b->power_alpha[k] : 0.0);
+            }
+            bool has_fixed = false;
+            for (int i = st; b->is_fixed && i < st + len; ++i) has_fixed |= b->is_fixed[i] != 0;
+            if (has_fixed) {
+                sec_start.push_back(st);
+                sec_vdim.push_back(vd);
+                sec_type.push_back((int)t);
+                sec_alpha.push_back(t == CONE_POWER && b->power_alpha ? b->power_alpha[k] : 0.0);
+                continue;
+            }
             const int li = t == CONE_ROTATED_SOC ? 0 : t == CONE_STANDARD_SOC ? 1 : t == CONE_EXPONENTIAL ? 2 : 3;
             lists[li].start.push_back(st);
             lists[li].vdim.push_back(vd);
@@ -137,6 +158,20 @@ class DevConeSet {
             L.alpha = dupload<double>(h.alpha, h.alpha.size());
             L.warm = dzeros<double>(h.start.size());
             L.rwarm = dzeros<double>(h.start.size());
+        }
+        if (!pall_start.empty()) {
+            powall.count = (int)pall_start.size();
+            powall.start = dupload<int>(pall_start, pall_start.size());
+            powall.alpha = dupload<double>(pall_alpha, pall_alpha.size());
+        }
+        if (!sec_start.empty()) {
+            sec.count = (int)sec_start.size();
+            sec.start = dupload<int>(sec_start, sec_start.size());
+            sec.vdim = dupload<int>(sec_vdim, sec_vdim.size());
+            sec.type = dupload<int>(sec_type, sec_type.size());
+            sec.alpha = dupload<double>(sec_alpha, sec_alpha.size());
+            std::vector<char> hf(b->is_fixed, b->is_fixed + n);
+            fixed = dupload<char>(hf, hf.size());
         }
         std::sort(ps.begin(), ps.end());
         for (size_t i = 0; i < ps.size();) {
@@ -175,6 +210,9 @@ class DevConeSet {
         if (powc.count)
             hipLaunchKernelGGL((cone_project_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v, d,
                                powc, rw, gate);
+        if (sec.count)
+            hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, (const X *)nullptr,
+                               0.0, fixed, sec, gate);
         for (const PsdGroup &g : groups)
             hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, v, g.starts, g.order, g.work,
                                gate);
@@ -195,14 +233,21 @@ class DevConeSet {
         if (powc.count)
             hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v,
                                d, q, tau, powc);
+        if (sec.count)
+            hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, q, tau, fixed,
+                               sec, (const int *)nullptr);
     }
 
     // Dual-cone residuals (rc - P_{K*}(rc)) d of rc = c + qx - aty (qx may be
     // null) written over dres on every cone slot; scratch (n entries) holds the
-    // PSD projections (reference src/cone_dispatch.cu:88-115).
+    // PSD projections (reference src/cone_dispatch.cu:88-115).  Blocks with
+    // fixed slots get the prox residual of their section at the point x.
     template <class X>
-    void dual_residual(double *dres, X *scratch, const X *c, const X *qx, const X *aty, const double *d, int n,
-                       hipStream_t st) const {
+    void dual_residual(double *dres, X *scratch, const X *x, const X *c, const X *qx, const X *aty, const double *d,
+                       int n, hipStream_t st) const {
+        if (sec.count)
+            hipLaunchKernelGGL(section_residual_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, dres, x, c, qx, aty,
+                               d, fixed, sec);
         if (rsoc.count)
             hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0,
                                st, dres, c, qx, aty, d, rsoc);
@@ -227,12 +272,16 @@ class DevConeSet {
     }
 
     void release() {
-        for (ConeList *L : {&soc, &rsoc, &expc, &powc}) {
+        for (ConeList *L : {&soc, &rsoc, &expc, &powc, &powall}) {
             void *lp[] = {L->start, L->vdim, L->alpha, L->warm, L->rwarm};
             for (void *p : lp)
                 if (p) (void)hipFree(p);
             *L = ConeList();
         }
+        for (void *p : {(void *)sec.start, (void *)sec.vdim, (void *)sec.type, (void *)sec.alpha, (void *)fixed})
+            if (p) (void)hipFree(p);
+        sec = SectionList();
+        fixed = nullptr;
         for (PsdGroup &g : groups) {
             if (g.starts) (void)hipFree(g.starts);
             if (g.work) (void)hipFree(g.work);
@@ -578,13 +627,15 @@ template <class V, class X> void Solver<V, X>::setup_cones(const ScaledProblem &
     const cone_blocks_t *b = sp.cones;
     if (!b || b->num_cones <= 0) return;
     cones_.build(b, n_);
+    // fixed slots keep their start value (reference src/solver_state.cu:650-651,673)
+    auto free_slot = [&](int i) { return !(b->is_fixed && b->is_fixed[i]); };
     for (int k = 0; k < b->num_cones; ++k) {
         const int aux = b->start_idx[k] + b->v_dim[k];
         if (b->type[k] == CONE_STANDARD_SOC) {
-            xs[aux] = -kInvSqrt2 * sp.cbr * sp.var_rescale[aux];
-            xs[(size_t)aux + 1] = kInvSqrt2 * sp.cbr * sp.var_rescale[(size_t)aux + 1];
+            if (free_slot(aux)) xs[aux] = -kInvSqrt2 * sp.cbr * sp.var_rescale[aux];
+            if (free_slot(aux + 1)) xs[(size_t)aux + 1] = kInvSqrt2 * sp.cbr * sp.var_rescale[(size_t)aux + 1];
         } else if (b->type[k] == CONE_ROTATED_SOC) {
-            xs[(size_t)aux + 1] = sp.cbr * sp.var_rescale[(size_t)aux + 1];
+            if (free_slot(aux + 1)) xs[(size_t)aux + 1] = sp.cbr * sp.var_rescale[(size_t)aux + 1];
         }
     }
     has_cones_ = true;
@@ -1061,7 +1112,7 @@ template <class V, class X> void Solver<V, X>::residual() {
     if (has_cones_) {
         // red[12..13] augmented dual residual, [14..17] power violation, [18] cone slack correction
         const X *qx = is_lp_ ? nullptr : qx_;
-        cones_.dual_residual(dres_, pgp_, c_, qx, aty_, var_rescale_, n_, st_);
+        cones_.dual_residual(dres_, pgp_, xp_, c_, qx, aty_, var_rescale_, n_, st_);
         // projected-gradient mapping (reference src/pdhg_core_op.cu:67-90)
         double t = step_ / pw_;
         if (!(t > 0.0) || !std::isfinite(t)) t = 1.0;
@@ -1072,9 +1123,9 @@ template <class V, class X> void Solver<V, X>::residual() {
                            var_rescale_, t, n_, r0);
         fold<1, 1>(grid(n_), 12);
         if (has_power_) {
-            hipLaunchKernelGGL(power_violation_kernel<X>, dim3(grid(cones_.powc.count)), dim3(kBlock), 0, st_, xp_,
-                               var_rescale_, cones_.powc, cbr_, r0);
-            fold<2, 2>(grid(cones_.powc.count), 14);
+            hipLaunchKernelGGL(power_violation_kernel<X>, dim3(grid(cones_.powall.count)), dim3(kBlock), 0, st_, xp_,
+                               var_rescale_, cones_.powall, cbr_, r0);
+            fold<2, 2>(grid(cones_.powall.count), 14);
         }
         hipLaunchKernelGGL(cone_slack_correction_kernel<X>, dim3(grid(n_)), dim3(kBlock), 0, st_, c_, qx, aty_,
                            dslack_, xp_, cones_.mask, n_, r0);
@@ -1346,12 +1397,6 @@ pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_paramet
         std::fprintf(stderr, "[interface] pdhcg_solver_create: invalid arguments.\n");
         return nullptr;
     }
-    for (int i = 0; prob->cones.is_fixed && i < prob->cones.fixed_mask_size; ++i)
-        if (prob->cones.is_fixed[i]) {
-            std::fprintf(stderr, "[interface] fixed cone sections are not supported by this build's device solver.\n");
-            return nullptr;
-        }
-
     if (prob->num_quadratic_constraints > 0) {
         std::fprintf(stderr, "Error: QCQP -> SOCP transformation is not available; cannot solve.\n");
         return nullptr;
@@ -1461,6 +1506,8 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
         HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         DevConeSet set;
         set.build(cones, n);
+        if (dual && set.sec.count)
+            throw std::runtime_error("blocks with fixed slots have no dual-cone residual without a primal point");
         std::vector<double> dh((size_t)n, 1.0);
         if (d) std::copy(d, d + n, dh.begin());
         std::vector<double> vh(in, in + n);
@@ -1469,7 +1516,7 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
         if (dual) {
             res = dupload<double>(std::vector<double>(out, out + n), n);
             scratch = dzeros<double>(n);
-            set.dual_residual<double>(res, scratch, v, nullptr, zero, dd, n, st);
+            set.dual_residual<double>(res, scratch, zero, v, nullptr, zero, dd, n, st);
         } else {
             set.project<double>(v, dd, false, nullptr, st);
         }
