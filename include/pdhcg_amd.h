@@ -74,6 +74,21 @@ int pdhcg_solver_kernel_time(pdhcg_solver_t *solver, int kernel, double *total_m
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n);
 int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n);
 
+/* Batched solve; the reference solves one problem per call and has no
+ * counterpart.  `count` independent problems are advanced together on the
+ * device, one workgroup per problem, each running the same restarted Halpern
+ * PDHG loop as solve_qp_problem (FP64) with its own step size, primal weight,
+ * restarts and termination.  results[i] receives problem i's result (free it
+ * with pdhcg_result_free), or NULL when problem i has something the batch does
+ * not handle (message on stderr): variable cone blocks, a sparse Q with
+ * off-diagonal entries or of type PDHCG_SPARSE_Q, quadratic constraints,
+ * affine PSD cones, affine second-order cones above 64 vector slots, a
+ * low-rank term above rank 64.  params (NULL: defaults) apply to every
+ * problem; the time limit is checked between windows of iterations.  Returns
+ * 0, or -1 for invalid arguments or a device error. */
+int pdhcg_solve_batch(const qp_problem_t *const *problems, int count, const pdhg_parameters_t *params,
+                      pdhcg_result_t **results);
+
 #ifdef __cplusplus
 }
 #endif

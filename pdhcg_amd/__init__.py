@@ -54,6 +54,9 @@ def lib() -> C.CDLL:
         L.pdhcg_solver_timing.argtypes = [C.c_void_p, C.c_int]
         L.pdhcg_solver_kernel_time.restype = C.c_int
         L.pdhcg_solver_kernel_time.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_int)]
+        L.pdhcg_solve_batch.restype = C.c_int
+        L.pdhcg_solve_batch.argtypes = [P(P(capi.qp_problem_t)), C.c_int, P(capi.pdhg_parameters_t),
+                                        P(P(capi.pdhcg_result_t))]
         L.pdhcg_project_cones.restype = C.c_int
         L.pdhcg_project_cones.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p, C.c_int]
         L.pdhcg_cone_dual_residual.restype = C.c_int
@@ -68,6 +71,37 @@ def solve(p: QPData, precision: int = PRECISION_FP64, **params) -> Optional[capi
     L = lib()
     L.pdhcg_set_precision(precision)
     return _solve(L, "", p, default_params(L, "", **params))
+
+
+def solve_batch(problems, **params) -> list:
+    """Solves the problems together through pdhcg_solve_batch (one device
+    workgroup per problem); returns one Solution per problem, or None for a
+    problem the batch rejected."""
+    L = lib()
+    handles = [build(L, "", p) for p in problems]
+    try:
+        count = len(problems)
+        ptrs = (C.POINTER(capi.qp_problem_t) * max(count, 1))()
+        for i, h in enumerate(handles):
+            if h is not None:
+                ptrs[i] = h.ptr
+        res = (C.POINTER(capi.pdhcg_result_t) * max(count, 1))()
+        prm = default_params(L, "", **params)
+        if L.pdhcg_solve_batch(ptrs, count, C.byref(prm), res) != 0:
+            raise RuntimeError("pdhcg_solve_batch failed")
+        out = []
+        for i, h in enumerate(handles):
+            if h is None or not res[i]:
+                out.append(None)
+                continue
+            n, m = h.contents.num_variables, h.contents.num_constraints
+            out.append(capi.result_to_solution(res[i], n, m if m > 0 else 0))
+            L.pdhcg_result_free(res[i])
+        return out
+    finally:
+        for h in handles:
+            if h is not None:
+                h.close()
 
 
 def project_cones(specs, x, d=None, fixed=None) -> np.ndarray:

@@ -1,0 +1,420 @@
+// Host side of the batched solve (pdhcg_solve_batch): each problem is checked,
+// scaled on the host exactly as solve_qp_problem scales it, and packed into
+// pooled device arrays; the device loop of dev_batch.hpp then advances every
+// unfinished problem by a window of iterations per launch until all have
+// terminated (or the batch's time limit is reached between windows).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pdhcg.h"
+#include "../../include/pdhcg_amd.h"
+#include "dev_batch.hpp"
+#include "host_scale.h"
+
+namespace pdhcg {
+namespace batch {
+namespace {
+
+// Loop state of one problem, persisted between launches (POD, same layout
+// on host and device).
+struct HostState {
+    int total, inner_count, inner_total, is_major, do_restart, reason;
+    double pw, pw_err_sum, pw_last_err, best_pw, best_gap;
+    double fpe, init_fpe, last_trial_fpe, inner_tol, cached_inv_tau, tol_scale;
+    double abs_pr, abs_du, rel_pr, rel_du, pobj, dobj, gap, rel_gap;
+    double max_pray, max_dray, pray_lin_obj, dray_obj;
+};
+static_assert(sizeof(HostState) == sizeof(State), "host and device loop state differ");
+
+// sigma_max(A) by the power iteration on A A' the single-problem solver uses
+// (reference src/pdhg_core_op.cu:1673-1788): start rand_r(1234) in [-1, 1].
+double host_max_sv(const HostCSR &A, const HostCSR &At, int maxit, double tol) {
+    const int m = A.rows, n = A.cols;
+    std::vector<double> e((size_t)m), ne((size_t)m), dp((size_t)n);
+    unsigned int seed = 1234;
+    for (int i = 0; i < m; ++i) e[i] = ((double)rand_r(&seed) / (double)RAND_MAX) * 2.0 - 1.0;
+    auto mv = [](const HostCSR &M, const std::vector<double> &x, std::vector<double> &y) {
+        for (int r = 0; r < M.rows; ++r) {
+            double s = 0.0;
+            for (int q = M.rp[r]; q < M.rp[(size_t)r + 1]; ++q) s += M.v[q] * x[M.ci[q]];
+            y[r] = s;
+        }
+    };
+    auto nrm = [](const std::vector<double> &v) {
+        double s = 0.0;
+        for (double a : v) s += a * a;
+        return std::sqrt(s);
+    };
+    double s2 = 1.0, inv = 1.0 / nrm(e);
+    for (double &a : e) a *= inv;
+    for (int it = 0; it < maxit; ++it) {
+        mv(At, e, dp);
+        mv(A, dp, ne);
+        s2 = 0.0;
+        for (int i = 0; i < m; ++i) s2 += ne[i] * e[i];
+        for (int i = 0; i < m; ++i) ne[i] += -s2 * e[i];
+        if (nrm(ne) < tol) break;
+        for (int i = 0; i < m; ++i) ne[i] += s2 * e[i];
+        inv = 1.0 / nrm(ne);
+        for (double &a : ne) a *= inv;
+        std::swap(e, ne);
+    }
+    return std::sqrt(s2);
+}
+
+struct Packed {
+    Inst inst;
+    ScaledProblem sp;
+    std::vector<int> ai;     // A rp, A ci, At rp, At ci
+    std::vector<double> av;  // A v, At v, R dense, D
+    std::vector<double> slab;
+    std::vector<Cone> cones;
+    HostState st;
+};
+
+// Problems the batch takes; a message names what is not supported.
+const char *unsupported(const qp_problem_t *p) {
+    if (!p) return "problem is NULL";
+    if (p->cones.num_cones > 0) return "variable cone blocks (use solve_qp_problem)";
+    if (p->num_quadratic_constraints > 0) return "quadratic constraints";
+    if (p->num_rank_lowrank_obj > kMaxRank) return "a low-rank term of rank above 64";
+    for (int c = 0; c < p->affine_cones.num_cones; ++c) {
+        const cone_type_t ct = p->affine_cones.type[c];
+        if (ct == CONE_PSD) return "an affine PSD cone";
+        if ((ct == CONE_STANDARD_SOC || ct == CONE_ROTATED_SOC) && p->affine_cones.v_dim[c] > kMaxThreadCone)
+            return "an affine second-order cone of more than 64 vector slots";
+    }
+    return nullptr;
+}
+
+// Scales and packs one problem; returns false (with *why set) for an
+// objective the batch does not handle.
+bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const char **why) {
+    k.sp = scale_problem(P, *prob);
+    ScaledProblem &sp = k.sp;
+    if (sp.qtype == PDHCG_SPARSE_Q) {
+        *why = "a sparse Q (use solve_qp_problem)";
+        return false;
+    }
+    if (sp.qtype == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
+        for (int i = 0; i < sp.n; ++i)
+            for (int q = sp.Q.rp[i]; q < sp.Q.rp[(size_t)i + 1]; ++q)
+                if (sp.Q.ci[q] != i) {
+                    *why = "a sparse Q with off-diagonal entries (use solve_qp_problem)";
+                    return false;
+                }
+    const int n = sp.n, m = sp.m, r = sp.k;
+    Inst &I = k.inst;
+    I.n = n;
+    I.m = m;
+    I.k = 0;
+    I.cbr = sp.cbr;
+    I.ovr = sp.ovr;
+    I.obj_const = sp.obj_const;
+    I.obj_norm = sp.obj_norm;
+    I.bound_norm = sp.bound_norm;
+    I.dkind = (int)sp.dkind;
+    I.nonconvexity = 0.0;
+    std::vector<double> qd((size_t)n, 0.0), dh((size_t)n, 0.0);
+    if (sp.qtype == PDHCG_NON_Q) I.qkind = kQNone;
+    else if (sp.qtype == PDHCG_DIAG_Q) {
+        I.qkind = kQDiag;
+        qd = sp.qdiag;
+        double mn = 0.0;
+        for (double q : qd) mn = std::fmin(mn, q);
+        I.nonconvexity = mn;
+    } else {
+        I.qkind = kQLowRank;
+        I.k = r;
+        if (sp.qtype == PDHCG_LOW_RANK_PLUS_SPARSE_Q)
+            for (int i = 0; i < n; ++i)
+                for (int q = sp.Q.rp[i]; q < sp.Q.rp[(size_t)i + 1]; ++q) qd[i] += sp.Q.v[q];
+    }
+    // pools: [A rp | A ci | At rp | At ci] and [A v | At v | R | D]
+    const HostCSR At = transpose(sp.A);
+    I.a_rp = 0;
+    I.a_ci = m + 1;
+    I.at_rp = I.a_ci + sp.A.nnz();
+    I.at_ci = I.at_rp + n + 1;
+    k.ai.insert(k.ai.end(), sp.A.rp.begin(), sp.A.rp.end());
+    k.ai.insert(k.ai.end(), sp.A.ci.begin(), sp.A.ci.end());
+    k.ai.insert(k.ai.end(), At.rp.begin(), At.rp.end());
+    k.ai.insert(k.ai.end(), At.ci.begin(), At.ci.end());
+    I.a_long = sp.A.nnz() >= 16LL * std::max(m, 1);
+    I.at_long = At.nnz() >= 16LL * std::max(n, 1);
+    I.a_v = 0;
+    I.at_v = sp.A.nnz();
+    k.av.insert(k.av.end(), sp.A.v.begin(), sp.A.v.end());
+    k.av.insert(k.av.end(), At.v.begin(), At.v.end());
+    I.rdense = (long long)k.av.size();
+    if (I.qkind == kQLowRank) {
+        std::vector<double> Rd((size_t)r * n, 0.0);
+        for (int j = 0; j < r; ++j)
+            for (int q = sp.R.rp[j]; q < sp.R.rp[(size_t)j + 1]; ++q) Rd[(size_t)j * n + sp.R.ci[q]] += sp.R.v[q];
+        // Jacobi diagonal: diag(Q) + diag(R'DR) (reference src/solver_state.cu:280-370)
+        dh = qd;
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            if (sp.dkind == MiddleKind::Diag) {
+                for (int j = 0; j < r; ++j) acc += sp.ddiag[j] * Rd[(size_t)j * n + i] * Rd[(size_t)j * n + i];
+            } else if (sp.dkind == MiddleKind::Dense) {
+                for (int a = 0; a < r; ++a)
+                    for (int b = 0; b < r; ++b)
+                        acc += Rd[(size_t)a * n + i] * Rd[(size_t)b * n + i] * sp.ddense[(size_t)a * r + b];
+            } else {
+                for (int j = 0; j < r; ++j) acc += Rd[(size_t)j * n + i] * Rd[(size_t)j * n + i];
+            }
+            dh[i] += acc;
+        }
+        k.av.insert(k.av.end(), Rd.begin(), Rd.end());
+    }
+    I.dmat = (long long)k.av.size();
+    if (sp.dkind == MiddleKind::Diag) k.av.insert(k.av.end(), sp.ddiag.begin(), sp.ddiag.end());
+    if (sp.dkind == MiddleKind::Dense) k.av.insert(k.av.end(), sp.ddense.begin(), sp.ddense.end());
+    // affine cones on the rows
+    const cone_blocks_t *ac = sp.affine_cones;
+    I.ncones = (ac && m > 0) ? ac->num_cones : 0;
+    I.has_affine = I.ncones > 0;
+    for (int c = 0; c < I.ncones; ++c)
+        k.cones.push_back(Cone{(int)ac->type[c], ac->start_idx[c], ac->v_dim[c],
+                               ac->power_alpha ? ac->power_alpha[c] : 0.0});
+    // step size and primal weight (reference src/pdhg_core_op.cu:933-971)
+    if (sp.A.nnz() == 0) I.step = 1.0;
+    else {
+        const double sv = host_max_sv(sp.A, At, P.sv_max_iter, P.sv_tol);
+        I.step = sv < 1e-12 ? 1.0 : 0.998 / sv;
+    }
+    I.pw0 = P.bound_objective_rescaling ? 1.0 : (sp.obj_norm + 1.0) / (sp.bound_norm + 1.0);
+    // the slab (FP64), starting point projected onto the box
+    k.slab.assign((size_t)slab_size(n, m, I.k, I.ncones), 0.0);
+    auto N = [&](int w) { return k.slab.data() + (size_t)w * n; };
+    auto M = [&](int w) { return k.slab.data() + (size_t)kNVec * n + (size_t)w * m; };
+    for (int i = 0; i < n; ++i) {
+        N(vC)[i] = sp.c[i];
+        N(vLB)[i] = sp.lb[i];
+        N(vUB)[i] = sp.ub[i];
+        N(vVR)[i] = sp.var_rescale[i];
+        N(vQD)[i] = qd[i];
+        N(vDH)[i] = dh[i];
+        N(vLBF)[i] = std::isfinite(sp.lb[i]) ? sp.lb[i] : 0.0;
+        N(vUBF)[i] = std::isfinite(sp.ub[i]) ? sp.ub[i] : 0.0;
+        double x0 = sp.primal_start.empty() ? 0.0 : sp.primal_start[i] * sp.var_rescale[i] * sp.cbr;
+        x0 = std::fmax(sp.lb[i], std::fmin(x0, sp.ub[i]));
+        N(vX0)[i] = N(vX)[i] = N(vXP)[i] = N(vXR)[i] = x0;
+    }
+    for (int i = 0; i < m; ++i) {
+        M(wG)[i] = sp.g.empty() ? 0.0 : sp.g[i];
+        M(wCL)[i] = sp.cl[i];
+        M(wCU)[i] = sp.cu[i];
+        M(wCLF)[i] = std::isfinite(sp.cl[i]) ? sp.cl[i] : 0.0;
+        M(wCUF)[i] = std::isfinite(sp.cu[i]) ? sp.cu[i] : 0.0;
+        M(wCR)[i] = sp.con_rescale[i];
+        M(wAFD)[i] = 1.0 / sp.con_rescale[i];
+        const double y0 = sp.dual_start.empty() ? 0.0 : sp.dual_start[i] * sp.con_rescale[i] * sp.ovr;
+        M(wY0)[i] = M(wY)[i] = M(wYP)[i] = y0;
+    }
+    HostState &S = k.st;
+    std::memset(&S, 0, sizeof(S));
+    S.pw = S.best_pw = I.pw0;
+    S.best_gap = INFINITY;
+    S.last_trial_fpe = INFINITY;
+    S.inner_tol = P.inner_solver_parameters.initial_tolerance > 0 ? P.inner_solver_parameters.initial_tolerance : 1e-3;
+    S.cached_inv_tau = -1.0;
+    S.tol_scale = 1.0;
+    S.reason = TERMINATION_REASON_UNSPECIFIED;
+    return true;
+}
+
+template <class T> T *upload(const std::vector<T> &h) {
+    T *d = nullptr;
+    HIP_OK(hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+    if (!h.empty()) HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+}  // namespace
+}  // namespace batch
+}  // namespace pdhcg
+
+using namespace pdhcg;
+using namespace pdhcg::batch;
+
+extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count, const pdhg_parameters_t *params,
+                                 pdhcg_result_t **results) {
+    if (count < 0 || (count > 0 && (!problems || !results))) {
+        std::fprintf(stderr, "[interface] pdhcg_solve_batch: invalid arguments.\n");
+        return -1;
+    }
+    for (int i = 0; i < count; ++i) results[i] = nullptr;
+    if (count == 0) return 0;
+    pdhg_parameters_t P;
+    if (params) P = *params;
+    else set_default_parameters(&P);
+    std::vector<void *> dev;
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<Packed> pk;
+        std::vector<int> idx;  // problem index of each packed entry
+        pk.reserve((size_t)count);
+        for (int i = 0; i < count; ++i) {
+            const char *why = unsupported(problems[i]);
+            Packed k;
+            if (!why && pack(problems[i], P, k, &why)) {
+                pk.push_back(std::move(k));
+                idx.push_back(i);
+            } else {
+                std::fprintf(stderr, "[interface] pdhcg_solve_batch: problem %d has %s.\n", i, why);
+            }
+        }
+        const int B = (int)pk.size();
+        if (B == 0) return 0;
+        // pool the problems
+        std::vector<Inst> insts((size_t)B);
+        std::vector<int> ipool;
+        std::vector<double> vpool;
+        std::vector<Cone> cones;
+        std::vector<HostState> states((size_t)B);
+        long long slab_total = 0;
+        for (int b = 0; b < B; ++b) {
+            Inst I = pk[b].inst;
+            const int io = (int)ipool.size();
+            const long long vo = (long long)vpool.size();
+            I.a_rp += io;
+            I.a_ci += io;
+            I.at_rp += io;
+            I.at_ci += io;
+            I.a_v += vo;
+            I.at_v += vo;
+            I.rdense += vo;
+            I.dmat += vo;
+            I.cone0 = (int)cones.size();
+            I.vec = slab_total;
+            slab_total += (long long)pk[b].slab.size();
+            ipool.insert(ipool.end(), pk[b].ai.begin(), pk[b].ai.end());
+            vpool.insert(vpool.end(), pk[b].av.begin(), pk[b].av.end());
+            cones.insert(cones.end(), pk[b].cones.begin(), pk[b].cones.end());
+            insts[b] = I;
+            states[b] = pk[b].st;
+        }
+        if (ipool.size() > (size_t)INT32_MAX) fail("the batch's index pool exceeds 2^31 entries");
+        std::vector<double> slabs;
+        slabs.reserve((size_t)slab_total);
+        for (const Packed &p : pk) slabs.insert(slabs.end(), p.slab.begin(), p.slab.end());
+        Params dp;
+        dp.ef = std::max(1, P.termination_evaluation_frequency);
+        dp.iteration_limit = P.termination_criteria.iteration_limit;
+        dp.norm_inf = P.optimality_norm == NORM_TYPE_L_INF;
+        dp.inner_limit = std::max(1, P.inner_solver_parameters.iteration_limit);
+        dp.precond = P.diag_jacobi_precond ? 1 : 0;
+        dp.eps_opt = P.termination_criteria.eps_optimal_relative;
+        dp.eps_feas = P.termination_criteria.eps_feasible_relative;
+        dp.eps_inf = P.termination_criteria.eps_infeasible;
+        dp.rho = P.reflection_coefficient;
+        dp.inner_tol0 = P.inner_solver_parameters.initial_tolerance > 0 ? P.inner_solver_parameters.initial_tolerance
+                                                                         : 1e-3;
+        dp.inner_min_tol = P.inner_solver_parameters.min_tolerance > 0 ? P.inner_solver_parameters.min_tolerance : 1e-9;
+        dp.i_smooth = P.restart_params.i_smooth;
+        dp.k_p = P.restart_params.k_p;
+        dp.k_i = P.restart_params.k_i;
+        dp.k_d = P.restart_params.k_d;
+        dp.suff = P.restart_params.sufficient_reduction_for_restart;
+        dp.nec = P.restart_params.necessary_reduction_for_restart;
+        dp.art = P.restart_params.artificial_restart_threshold;
+        auto track = [&](void *p) {
+            dev.push_back(p);
+            return p;
+        };
+        Params *d_params = (Params *)track(upload(std::vector<Params>{dp}));
+        Inst *d_insts = (Inst *)track(upload(insts));
+        int *d_ipool = (int *)track(upload(ipool));
+        double *d_vpool = (double *)track(upload(vpool));
+        double *d_slabs = (double *)track(upload(slabs));
+        Cone *d_cones = (Cone *)track(upload(cones));
+        State *d_states = (State *)track(upload(states));
+        Out *d_out = (Out *)track(upload(std::vector<Out>((size_t)B)));
+        hipStream_t st;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        // windows of iterations; every problem that has terminated returns at once
+        const int window = std::max(1, std::min(dp.ef * 16, 2000));
+        std::vector<Out> out((size_t)B);
+        bool time_up = false;
+        for (;;) {
+            hipLaunchKernelGGL(batch_solve_kernel, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool, d_vpool,
+                               d_slabs, d_cones, d_states, d_out, window);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(Out), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            bool all = true;
+            for (const Out &o : out) all = all && o.status != TERMINATION_REASON_UNSPECIFIED;
+            if (all) break;
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el >= P.termination_criteria.time_sec_limit) {
+                time_up = true;
+                break;
+            }
+        }
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        HIP_OK(hipMemcpy(slabs.data(), d_slabs, slabs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        (void)hipStreamDestroy(st);
+        for (int b = 0; b < B; ++b) {
+            const Inst &I = insts[b];
+            const ScaledProblem &sp = pk[b].sp;
+            const Out &o = out[b];
+            const double *V = slabs.data() + I.vec;
+            const double *xp = V + (size_t)vXP * I.n, *aty = V + (size_t)vATY * I.n, *qx = V + (size_t)vQX * I.n;
+            const double *yp = V + (size_t)kNVec * I.n + (size_t)wYP * I.m;
+            pdhcg_result_t *r = ccalloc<pdhcg_result_t>(1);
+            r->primal_solution = cmalloc<double>(I.n);
+            r->dual_solution = cmalloc<double>(I.m);
+            r->reduced_cost = cmalloc<double>(I.n);
+            for (int j = 0; j < I.n; ++j) {  // reference src/pdhg_core_op.cu:1431-1496
+                double gr = sp.c[j];
+                if (I.qkind != kQNone) gr += qx[j];
+                double rc = (gr - aty[j]) * sp.var_rescale[j] / sp.ovr;
+                if (!std::isfinite(sp.lb[j])) rc = std::fmin(rc, 0.0);
+                if (!std::isfinite(sp.ub[j])) rc = std::fmax(rc, 0.0);
+                r->reduced_cost[j] = rc;
+                r->primal_solution[j] = xp[j] / sp.var_rescale[j] / sp.cbr;
+            }
+            for (int i = 0; i < I.m; ++i) r->dual_solution[i] = yp[i] / sp.con_rescale[i] / sp.ovr;
+            const qp_problem_t *orig = problems[idx[b]];
+            r->num_variables = orig->num_variables;
+            r->num_constraints = orig->num_constraints;
+            r->num_nonzeros = orig->constraint_matrix_num_nonzeros;
+            r->total_count = o.iters;
+            r->total_inner_count = o.inner;
+            r->rescaling_time_sec = sp.rescale_time;
+            r->cumulative_time_sec = elapsed;
+            r->relative_primal_residual = o.rel_pr;
+            r->relative_dual_residual = o.rel_du;
+            r->absolute_primal_residual = o.abs_pr;
+            r->absolute_dual_residual = o.abs_du;
+            r->primal_objective_value = o.pobj;
+            r->dual_objective_value = o.dobj;
+            r->objective_gap = o.gap;
+            r->relative_objective_gap = o.rel_gap;
+            r->max_primal_ray_infeasibility = o.max_pray;
+            r->max_dual_ray_infeasibility = o.max_dray;
+            r->primal_ray_linear_objective = o.pray_lin_obj;
+            r->dual_ray_objective = o.dray_obj;
+            r->termination_reason = o.status != TERMINATION_REASON_UNSPECIFIED
+                                        ? (termination_reason_t)o.status
+                                        : (time_up ? TERMINATION_REASON_TIME_LIMIT : TERMINATION_REASON_UNSPECIFIED);
+            results[idx[b]] = r;
+        }
+        for (void *p : dev) (void)hipFree(p);
+        return 0;
+    } catch (const std::exception &e) {
+        for (void *p : dev) (void)hipFree(p);
+        std::fprintf(stderr, "[interface] pdhcg_solve_batch: %s\n", e.what());
+        return -1;
+    }
+}

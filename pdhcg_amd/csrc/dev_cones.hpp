@@ -88,7 +88,7 @@ __device__ double soc_multiplier(Get dh, const double *pv, int k, double w, doub
 
 // Projects slots (v[0..k), a, b) held in FP64 registers/scratch `pv` (length
 // k + 2) in place.  rotated: (a, b) = (s, t), else (w, z).  d: block scaling.
-__device__ void soc_project_block(double *pv, int k, const double *d, bool rotated, double *warm) {
+__device__ inline void soc_project_block(double *pv, int k, const double *d, bool rotated, double *warm) {
     const double a = pv[k], b = pv[k + 1];
     const double w = rotated ? (a - b) * kInvSqrt2 : a;
     const double z = rotated ? (a + b) * kInvSqrt2 : b;
@@ -142,7 +142,7 @@ __device__ void soc_project_block(double *pv, int k, const double *d, bool rotat
 
 // Dual residual (rc - P_{K*}(rc)) d of a SOC/RSOC block; rc in `pr`
 // (length k + 2), result written to `out` (FP64).
-__device__ void soc_dual_residual_block(const double *pr, int k, const double *d, bool rotated, double *warm,
+__device__ inline void soc_dual_residual_block(const double *pr, int k, const double *d, bool rotated, double *warm,
                                         double *out) {
     const double a = pr[k], b = pr[k + 1];
     const double rw = rotated ? (a - b) * kInvSqrt2 : a;
@@ -223,7 +223,7 @@ __device__ void soc_dual_residual_block(const double *pr, int k, const double *d
 
 // ---------------- exponential cone ----------------
 
-__device__ void exp_project_point(double r1, double r2, double r3, double d1, double d2, double d3, double *o) {
+__device__ inline void exp_project_point(double r1, double r2, double r3, double d1, double d2, double d3, double *o) {
     const double E = 2.718281828459045;
     const double a1 = r1 / d1, a2 = r2 / d2, a3 = r3 / d3;
     if (a2 > 0.0) {
@@ -292,7 +292,7 @@ __device__ __forceinline__ double positive_root(double r, double q) {
     return r >= 0.0 ? 0.5 * (r + disc) : (2.0 * q) / (disc - r);
 }
 
-__device__ void power_project_normalized(double rx, double ry, double rz, double wx, double wy, double wz, double al,
+__device__ inline void power_project_normalized(double rx, double ry, double rz, double wx, double wy, double wz, double al,
                                          double *o) {
     const double az = fabs(rz), sg = rz >= 0.0 ? 1.0 : -1.0, om = 1.0 - al;
     if (az == 0.0) {
@@ -330,7 +330,7 @@ __device__ void power_project_normalized(double rx, double ry, double rz, double
     o[2] = sg * fmin(rho, exp(al * log(o[0]) + om * log(o[1])));
 }
 
-__device__ void power_project_point(double rx, double ry, double rz, double wx, double wy, double wz, double al,
+__device__ inline void power_project_point(double rx, double ry, double rz, double wx, double wy, double wz, double al,
                                     double *o) {
     const double sc = fmax(fabs(rx), fmax(fabs(ry), fabs(rz)));
     if (!(sc > 0.0) || !isfinite(sc)) {
@@ -702,7 +702,7 @@ __global__ void __launch_bounds__(kBlock) affine_rows_finish_kernel(const X *ax,
 }
 
 // out = [sum a^2, max |a|] of a double vector.
-__global__ void __launch_bounds__(kBlock) sq_max_kernel(const double *a, int n, GridRed gr) {
+static __global__ void __launch_bounds__(kBlock) sq_max_kernel(const double *a, int n, GridRed gr) {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v[0] += a[i] * a[i];

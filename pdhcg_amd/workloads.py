@@ -134,7 +134,7 @@ def random_problem(kind: str, seed: int = 0, n: int = 60, m: int = 30) -> QPData
     raise ValueError(kind)
 
 
-def portfolio_tracking(n: int, k: int, seed: int = 1003, te_factor: float = 1.5) -> QPData:
+def portfolio_tracking(n: int, k: int, seed=1003, te_factor: float = 1.5) -> QPData:
     """Long-only portfolio with a tracking-error limit as an affine SOC.
 
     min 1/2 x'(diag(q) + R' D R) x + c'x  s.t.  1'x = 1,  0 <= x <= 0.1,
@@ -155,3 +155,15 @@ def portfolio_tracking(n: int, k: int, seed: int = 1003, te_factor: float = 1.5)
     return QPData(c=c, A=sp.csr_matrix(np.ones((1, n))), con_lb=[1.0], con_ub=[1.0], var_lb=np.zeros(n),
                   var_ub=np.full(n, 0.1), Q=sp.diags(q).tocsr(), R=R, D=D, F=sp.csr_matrix(F), g=g,
                   affine_cones=[(1, 0, k)], fmt={"R": "dense"})
+
+
+def portfolio_batch(count: int, seed: int = 1003, n_range=(50, 2000), k_range=(10, 50),
+                    te_factor: float = 1.5) -> list:
+    """The batched portfolio workload: `count` independent instances of
+    portfolio_tracking with n ~ U{n_range} assets and k ~ U{k_range} factors;
+    instance i draws its data from the seed pair (seed, i)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    ks = rng.integers(k_range[0], k_range[1] + 1, size=count)
+    return [portfolio_tracking(int(n), int(k), seed=(seed, i), te_factor=te_factor)
+            for i, (n, k) in enumerate(zip(ns, ks))]

@@ -1,0 +1,85 @@
+"""GPU parity of the batched solve (pdhcg_solve_batch): every problem of a
+batch of uneven sizes and kinds against the oracle's individual solve of the
+same problem.
+
+Bar (as for single solves): same termination status, objective within
+5 eps (1 + |obj|) of the oracle's, the independent numpy KKT check at the
+solve tolerance for problems without affine cones, and for the portfolios
+the budget row and the tracking-error limit.  Problems outside the batch's
+scope come back as None while the rest of the batch is solved.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import pdhcg_amd
+from pdhcg_amd import QPData, workloads
+from tests.kkt import kkt
+from tests.oracle_lib import oracle_solve
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+
+
+def _agree(p, g, o, eps=EPS, affine=False):
+    assert g is not None and o is not None
+    assert g.status == o.status, (g.status, o.status)
+    if g.status != "OPTIMAL":
+        return
+    assert abs(g.pobj - o.pobj) <= 5 * eps * (1 + abs(o.pobj)), (g.pobj, o.pobj)
+    if not affine:
+        k = kkt(p, g.x, g.y)
+        assert k["rel_primal"] < 10 * eps and k["rel_dual"] < 10 * eps and k["rel_gap"] < 10 * eps, k
+
+
+def test_batch_of_mixed_kinds_matches_oracle():
+    # uneven sizes; each case converges for the oracle in a few thousand steps
+    cases = [("lp", 40, 0), ("lp", 51, 1), ("diag", 77, 0), ("diag", 88, 1), ("lowrank", 80, 1), ("lowrank", 114, 0),
+             ("lowrank_diag", 151, 0), ("lowrank_diag", 162, 1), ("lowrank_dense_d", 60, 1),
+             ("lowrank_dense_d", 70, 0), ("lp", 150, 3), ("lowrank_diag", 200, 2)]
+    problems = [workloads.random_problem(kind, seed=seed, n=n, m=max(10, n // 3)) for kind, n, seed in cases]
+    sols = pdhcg_amd.solve_batch(problems, eps=EPS, iteration_limit=200000, time_limit=60.0)
+    for p, g in zip(problems, sols):
+        _agree(p, g, oracle_solve(p, eps=EPS, iteration_limit=200000))
+
+
+def test_batch_portfolios_match_oracle():
+    problems = workloads.portfolio_batch(6, seed=11, n_range=(50, 400), k_range=(10, 50))
+    sols = pdhcg_amd.solve_batch(problems, eps=EPS, iteration_limit=200000, time_limit=60.0)
+    for p, g in zip(problems, sols):
+        o = oracle_solve(p, eps=EPS, iteration_limit=200000)
+        _agree(p, g, o, affine=True)
+        te = np.linalg.norm(np.sqrt(p.D) * (p.R @ g.x))
+        assert te <= p.g[-1] * (1 + 1e-4)
+        assert abs(g.x.sum() - 1.0) <= 1e-5
+
+
+def test_batch_rejects_unsupported_and_solves_the_rest():
+    good = workloads.random_problem("lp", seed=3, n=50, m=20)
+    sparse_q = workloads.random_problem("sparse", seed=3, n=50, m=20)  # off-diagonal Q
+    sols = pdhcg_amd.solve_batch([good, sparse_q, good], iteration_limit=200000, time_limit=60.0)
+    assert sols[1] is None
+    assert sols[0] is not None and sols[2] is not None
+    assert sols[0].status == sols[2].status == "OPTIMAL"
+    assert sols[0].pobj == sols[2].pobj  # a problem's trajectory does not depend on the batch
+
+
+def test_batch_infeasible_lp_and_iteration_limit():
+    infeasible = QPData(c=[1.0, 1.0], A=sp.csr_matrix([[1.0, 1.0]]), con_lb=[3.0], con_ub=[np.inf],
+                        var_lb=[0.0, 0.0], var_ub=[1.0, 1.0])
+    slow = workloads.random_problem("lowrank_diag", seed=5, n=80, m=30)
+    sols = pdhcg_amd.solve_batch([infeasible, slow], iteration_limit=20000)
+    assert sols[0].status == oracle_solve(infeasible, iteration_limit=20000).status == "PRIMAL_INFEASIBLE"
+    capped = pdhcg_amd.solve_batch([slow], eps=1e-12, iteration_limit=57)[0]
+    o = oracle_solve(slow, eps=1e-12, iteration_limit=57)
+    assert capped.status == o.status == "ITERATION_LIMIT"
+    assert capped.iterations == o.iterations == 57
+
+
+def test_batch_matches_single_device_solve():
+    p = workloads.portfolio_tracking(300, 20, seed=21)
+    b = pdhcg_amd.solve_batch([p], eps=EPS, iteration_limit=200000, time_limit=60.0)[0]
+    s = pdhcg_amd.solve(p, eps=EPS, iteration_limit=200000)
+    assert b.status == s.status == "OPTIMAL"
+    assert abs(b.pobj - s.pobj) <= 5 * EPS * (1 + abs(s.pobj))
