@@ -155,6 +155,54 @@ def run_cpu_baseline(p, eps, budget_s):
     return dict(iters=iters, seconds=dt, setup_s=setup)
 
 
+def run_batch(count, eps, cpu_seconds, with_cpu):
+    """The batched portfolio workload: `count` instances (n ~ U{50..2000},
+    k ~ U{10..50}, one tracking-error SOC each) solved by pdhcg_solve_batch,
+    beside the single-problem device solve of a sample of the same instances
+    and, optionally, the FP64 CPU oracle on every 64th instance (bounded)."""
+    import numpy as np
+    import pdhcg_amd
+    from pdhcg_amd import workloads
+    t = time.perf_counter()
+    problems = workloads.portfolio_batch(count, seed=1003)
+    gen_s = time.perf_counter() - t
+    log(f"[bench] batch: {count} portfolio instances generated in {gen_s:.1f}s")
+    t = time.perf_counter()
+    sols = pdhcg_amd.solve_batch(problems, eps=eps, iteration_limit=100000, time_limit=300.0, verbose=1)
+    batch_s = time.perf_counter() - t
+    status = {}
+    for s_ in sols:
+        k = s_.status if s_ is not None else "REJECTED"
+        status[k] = status.get(k, 0) + 1
+    iters = [s_.iterations for s_ in sols if s_ is not None]
+    log(f"[bench] batch: solved in {batch_s:.2f}s, {status}")
+    sample = problems[:: max(1, count // 8)][:8]
+    t = time.perf_counter()
+    single = [pdhcg_amd.solve(p_, eps=eps, iteration_limit=100000) for p_ in sample]
+    single_ms = 1e3 * (time.perf_counter() - t) / len(sample)
+    out = {"workload": "%d long-only portfolio QPs, n ~ U{50..2000} (total n = %d), factor rank k ~ U{10..50}, "
+                       "budget row, box [0, 0.1], tracking-error SOC; diag Q + R'DR, FP64" % (
+                           count, sum(p_.n for p_ in problems)),
+           "problems": count, "time_to_tol_s": batch_s, "ms_per_problem": 1e3 * batch_s / count,
+           "status": status, "iterations_mean": float(np.mean(iters)), "iterations_max": int(max(iters)),
+           "single_device_ms_per_problem": single_ms, "single_sample": len(sample),
+           "speedup_vs_single_device": single_ms / (1e3 * batch_s / count),
+           "note": "time_to_tol_s covers host scaling and packing plus the device solve of the whole batch"}
+    if with_cpu:
+        from tests.oracle_lib import oracle_solve
+        t = time.perf_counter()
+        done = 0
+        for p_ in problems[::64]:
+            oracle_solve(p_, eps=eps, iteration_limit=100000)
+            done += 1
+            if time.perf_counter() - t > cpu_seconds:
+                break
+        cpu_ms = 1e3 * (time.perf_counter() - t) / max(done, 1)
+        out["cpu_baseline"] = {"ms_per_problem": cpu_ms, "cores": 1, "kind": "port",
+                               "sample": f"FP64 oracle on {done} of every 64th instance to 1e-6, single thread"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +215,8 @@ def main():
     ap.add_argument("--max-solve-seconds", type=float, default=60.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096, metavar="N",
+                    help="problems in the batched portfolio workload (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy used to calibrate rocprofv3 byte counters")
     args = ap.parse_args()
@@ -263,6 +313,8 @@ def main():
         out["other_precision"] = {"precision": other, "iter_per_s": args.steps / o["step_s"],
                                   "time_to_tol_s": o["time_to_tol"], "status": o["status"],
                                   "iterations": o["total_iters"], "rel_residuals": o["rel"], "pobj": o["pobj"]}
+    if args.batch > 0:
+        out["batched"] = run_batch(args.batch, args.eps, args.cpu_seconds, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -6,12 +6,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pdhcg.h"
@@ -261,19 +264,41 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
     std::vector<void *> dev;
     try {
         const auto t0 = std::chrono::steady_clock::now();
-        std::vector<Packed> pk;
-        std::vector<int> idx;  // problem index of each packed entry
-        pk.reserve((size_t)count);
-        for (int i = 0; i < count; ++i) {
-            const char *why = unsupported(problems[i]);
-            Packed k;
-            if (!why && pack(problems[i], P, k, &why)) {
-                pk.push_back(std::move(k));
-                idx.push_back(i);
-            } else {
-                std::fprintf(stderr, "[interface] pdhcg_solve_batch: problem %d has %s.\n", i, why);
-            }
+        // scale and pack the problems on all host cores (they are independent)
+        std::vector<Packed> all((size_t)count);
+        std::vector<const char *> why((size_t)count, nullptr);
+        std::vector<std::string> err((size_t)count);
+        {
+            const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::atomic<int> next{0};
+            auto work = [&]() {
+                for (int i = next++; i < count; i = next++) {
+                    try {
+                        why[i] = unsupported(problems[i]);
+                        if (!why[i]) pack(problems[i], P, all[i], &why[i]);
+                    } catch (const std::exception &e) {
+                        err[i] = e.what();
+                    }
+                }
+            };
+            std::vector<std::thread> pool;
+            for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+            work();
+            for (std::thread &t : pool) t.join();
         }
+        for (int i = 0; i < count; ++i)
+            if (!err[i].empty()) fail("problem %d: %s", i, err[i].c_str());
+        // largest problems first, so they start in the first round of workgroups
+        std::vector<int> idx;  // problem index of each packed entry
+        for (int i = 0; i < count; ++i) {
+            if (why[i]) std::fprintf(stderr, "[interface] pdhcg_solve_batch: problem %d has %s.\n", i, why[i]);
+            else idx.push_back(i);
+        }
+        auto work_of = [&](int i) { return (double)all[i].inst.n * (1.0 + all[i].inst.k) + all[i].inst.m; };
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return work_of(a) > work_of(b); });
+        std::vector<Packed> pk;
+        pk.reserve(idx.size());
+        for (int i : idx) pk.push_back(std::move(all[i]));
         const int B = (int)pk.size();
         if (B == 0) return 0;
         // pool the problems
@@ -342,6 +367,8 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         Out *d_out = (Out *)track(upload(std::vector<Out>((size_t)B)));
         hipStream_t st;
         HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        const auto t_dev = std::chrono::steady_clock::now();
+        int windows = 0;
         // windows of iterations; every problem that has terminated returns at once
         const int window = std::max(1, std::min(dp.ef * 16, 2000));
         std::vector<Out> out((size_t)B);
@@ -349,6 +376,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         for (;;) {
             hipLaunchKernelGGL(batch_solve_kernel, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool, d_vpool,
                                d_slabs, d_cones, d_states, d_out, window);
+            ++windows;
             HIP_OK(hipGetLastError());
             HIP_OK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(Out), hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
@@ -361,7 +389,12 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                 break;
             }
         }
-        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const auto t_end = std::chrono::steady_clock::now();
+        const double elapsed = std::chrono::duration<double>(t_end - t0).count();
+        if (P.verbose >= 1)
+            std::fprintf(stderr, "[batch] %d problems: host scaling and packing %.3fs, device %.3fs in %d windows\n", B,
+                         std::chrono::duration<double>(t_dev - t0).count(),
+                         std::chrono::duration<double>(t_end - t_dev).count(), windows);
         HIP_OK(hipMemcpy(slabs.data(), d_slabs, slabs.size() * sizeof(double), hipMemcpyDeviceToHost));
         (void)hipStreamDestroy(st);
         for (int b = 0; b < B; ++b) {

@@ -8,6 +8,10 @@
 // reduction, so a problem's trajectory does not depend on the rest of the
 // batch.  Scalars are held by every thread (each thread computes the same
 // value from the same broadcast reductions), so the control flow is uniform.
+// The once-per-iteration helpers are not inlined: each gets its own register
+// allocation, which keeps the kernel at 248 VGPRs without spills, i.e. two
+// waves per SIMD with 512 threads per problem (fully inlined at 256 threads
+// it needed 294 registers and ran one problem per CU).
 //
 // Supported per problem: box bounds, rows [A; F] in CSR, objective of type
 // LP, diagonal Q, or dense low rank R'DR (k <= 64) plus a diagonal part, and
@@ -24,7 +28,7 @@
 namespace pdhcg {
 namespace batch {
 
-constexpr int kBT = 256;  // threads per problem
+constexpr int kBT = 512;  // threads per problem (two waves per SIMD at 248 VGPRs)
 constexpr int kBW = kBT / kWave;
 constexpr int kMaxRank = 64;
 
@@ -166,7 +170,7 @@ struct Ctx {
     __device__ void At(const double *y, double *x) const { spmv(atrp, atci, atv, y, x, I.n, I.at_long); }
 
     // qx = Q x (reference src/pdhg_core_op.cu:305-413): diagonal part plus R'DR x
-    __device__ void obj_product(const double *x, double *qx) const {
+    __device__ __noinline__ void obj_product(const double *x, double *qx) const {
         const int n = I.n, k = I.k;
         if (I.qkind == kQNone) return;
         const double *qd = nv(vQD);
@@ -212,7 +216,7 @@ struct Ctx {
 
     // Projection of the m-vector v onto the affine cones, scaling affd
     // (reference src/cone_dispatch.cu:38-57 on the row space).
-    __device__ void project_affine(double *v, double *warm) const {
+    __device__ __noinline__ void project_affine(double *v, double *warm) const {
         const double *d = mv(wAFD);
         for (int c = threadIdx.x; c < I.ncones; c += kBT) {
             const Cone cn = cones[c];
@@ -251,7 +255,7 @@ struct State {
 };
 
 // Projected BB inner solve of the primal prox (reference src/pdhg_core_op.cu:495-669).
-__device__ inline void bb_prox(const Ctx &C, State &S, double tau) {
+__device__ __noinline__ void bb_prox(const Ctx &C, State &S, double tau) {
     const Params &P = *C.P;
     const int n = C.I.n;
     double *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *grad = C.nv(vGR), *dirn = C.nv(vDIR), *qx = C.nv(vQX),
@@ -321,7 +325,7 @@ __device__ inline void bb_prox(const Ctx &C, State &S, double tau) {
 }
 
 // One PDHG step (reference src/pdhg_core_op.cu:705-830).
-__device__ inline void pdhg_step(const Ctx &C, State &S) {
+__device__ __noinline__ void pdhg_step(const Ctx &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     double tau = I.step / S.pw;
@@ -398,7 +402,7 @@ __device__ inline void halpern(const Ctx &C, const State &S) {
 }
 
 // reference src/pdhg_core_op.cu:973-1035
-__device__ inline void fixed_point_error(const Ctx &C, State &S) {
+__device__ __noinline__ void fixed_point_error(const Ctx &C, State &S) {
     const int n = C.I.n, m = C.I.m;
     double *dx = C.nv(vDX), *dy = C.mv(wDY), *aty = C.nv(vATY);
     const double *xr = C.nv(vXR), *x = C.nv(vX), *yr = C.mv(wYR), *y = C.mv(wY);
@@ -416,7 +420,7 @@ __device__ inline void fixed_point_error(const Ctx &C, State &S) {
 }
 
 // reference src/pdhg_core_op.cu:859-931
-__device__ inline void restart(const Ctx &C, State &S) {
+__device__ __noinline__ void restart(const Ctx &C, State &S) {
     const Params &P = *C.P;
     const int n = C.I.n, m = C.I.m;
     double *x0 = C.nv(vX0), *x = C.nv(vX), *y0 = C.mv(wY0), *y = C.mv(wY);
@@ -463,7 +467,7 @@ __device__ inline bool should_restart(const Ctx &C, State &S) {
 }
 
 // KKT residuals (reference src/pdhg_core_op.cu:1037-1294).
-__device__ inline void residual(const Ctx &C, State &S) {
+__device__ __noinline__ void residual(const Ctx &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     const bool linf = C.P->norm_inf != 0;
@@ -554,7 +558,7 @@ __device__ inline void residual(const Ctx &C, State &S) {
 }
 
 // Infeasibility rays (reference src/pdhg_core_op.cu:1296-1429).
-__device__ inline void infeasibility(const Ctx &C, State &S) {
+__device__ __noinline__ void infeasibility(const Ctx &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     double *dx = C.nv(vDX), *dy = C.mv(wDY), *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX);
