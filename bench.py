@@ -67,7 +67,7 @@ def run_gpu(p, precision, steps, warmup, eps, max_solve_s):
     L.pdhcg_solver_kernel_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     prec = pdhcg_amd.PRECISION_MIXED if precision == "mixed" else pdhcg_amd.PRECISION_FP64
     t = time.time()
-    st = pdhcg_amd.Stepper(p, prec, eps=eps)
+    st = pdhcg_amd.Stepper(p, prec, eps=eps, verbose=1)
     setup_s = time.time() - t
     log(f"[bench] {precision}: solver setup {setup_s:.1f}s")
     t_solve0 = time.perf_counter()

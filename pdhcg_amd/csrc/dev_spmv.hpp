@@ -1,7 +1,8 @@
 // Row-length-binned CSR sparse matrix-vector product for gfx950.
 //
 // Rows are sorted (stably) into classes by length.  A class assigns L lanes
-// of a 64-wide wavefront to each row (L = 1, 2, ..., 64), so short rows are
+// of a 64-wide wavefront to each row (L = 1, 2, ..., 64, shifted up or down
+// one power of two by the setup autotuner), so short rows are
 // packed many to a wave and long rows get a whole wave; rows longer than
 // kLongRow get a full 256-thread workgroup with an LDS reduction.  One launch
 // covers every class: the workgroup index selects the class through a small
@@ -40,7 +41,27 @@ inline int spmv_class_of(int len) {
 struct SpmvClassTable {
     int class_start[kSpmvClasses + 1];  // first slot of each class
     int block_start[kSpmvClasses + 1];  // first workgroup of each class
+    int shift;                          // class c < 7 runs 2^clamp(c + shift, 0, 6) lanes per row
 };
+
+// Lanes (as a power of two) of class c < 7 under a lane shift.
+__host__ __device__ inline int spmv_lane_log(int c, int shift) {
+    const int l = c + shift;
+    return l < 0 ? 0 : (l > 6 ? 6 : l);
+}
+
+// Workgroup offsets of the classes for a lane shift (counts = rows per class).
+inline void spmv_set_blocks(SpmvClassTable &tab, int shift) {
+    tab.shift = shift;
+    tab.block_start[0] = 0;
+    for (int c = 0; c < kSpmvClasses; ++c) {
+        const long long count = tab.class_start[c + 1] - tab.class_start[c];
+        long long blocks;
+        if (c == 7) blocks = count;
+        else blocks = (count * (1LL << spmv_lane_log(c, shift)) + kBlock - 1) / kBlock;
+        tab.block_start[c + 1] = tab.block_start[c] + (int)blocks;
+    }
+}
 
 // A CSR matrix in slot order: slot s holds original row row[s], whose
 // entries are ci/v[rp[s] .. rp[s + 1]).
@@ -75,13 +96,7 @@ BinnedCsr<V> make_binned(int rows, int cols, const std::vector<int> &rp, const s
     for (int c = 0; c < kSpmvClasses; ++c) b.tab.class_start[c + 1] = b.tab.class_start[c] + count[c];
     std::vector<int> order((size_t)rows), cursor(b.tab.class_start, b.tab.class_start + kSpmvClasses);
     for (int r = 0; r < rows; ++r) order[cursor[cls[r]]++] = r;
-    b.tab.block_start[0] = 0;
-    for (int c = 0; c < kSpmvClasses; ++c) {
-        long long blocks;
-        if (c == 7) blocks = count[c];
-        else blocks = ((long long)count[c] * (1LL << c) + kBlock - 1) / kBlock;
-        b.tab.block_start[c + 1] = b.tab.block_start[c] + (int)blocks;
-    }
+    spmv_set_blocks(b.tab, 0);
     b.total_blocks = b.tab.block_start[kSpmvClasses];
     std::vector<int> srp((size_t)rows + 1, 0);
     for (int s = 0; s < rows; ++s) srp[(size_t)s + 1] = srp[s] + (rp[(size_t)order[s] + 1] - rp[order[s]]);
@@ -101,6 +116,11 @@ BinnedCsr<V> make_binned(int rows, int cols, const std::vector<int> &rp, const s
     b.ci = upload_vec(sci);
     b.v = upload_vec(sv);
     return b;
+}
+
+template <class V> void set_lane_shift(BinnedCsr<V> &b, int shift) {
+    spmv_set_blocks(b.tab, shift);
+    b.total_blocks = b.tab.block_start[kSpmvClasses];
 }
 
 template <class V> void free_binned(BinnedCsr<V> &b) {
@@ -147,9 +167,10 @@ __device__ __forceinline__ void binned_block(const BinnedCsr<V> &M, const X *__r
     for (int k = 1; k < kSpmvClasses; ++k) c += (b >= tab.block_start[k]) ? 1 : 0;
     const int local_block = b - tab.block_start[c];
     if (c < 7) {
-        const int lanes = 1 << c;
+        const int ll = spmv_lane_log(c, tab.shift);
+        const int lanes = 1 << ll;
         const int t = local_block * kBlock + threadIdx.x;
-        const int slot = t >> c;
+        const int slot = t >> ll;
         const int lane = t & (lanes - 1);
         const int count = tab.class_start[c + 1] - tab.class_start[c];
         const bool active = slot < count;

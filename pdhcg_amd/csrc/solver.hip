@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <type_traits>
 #include <utility>
@@ -333,6 +334,9 @@ template <class V, class X> class Solver final : public SolverBase {
 
     // ---- setup helpers
     void upload(const ScaledProblem &sp);
+    void tune_spmv();
+    float time_launches(const std::function<void()> &launch);
+    int shift_A_ = 0, shift_At_ = 0;  // autotuned lane shifts of the A and A' products
     void setup_cones(const ScaledProblem &sp, std::vector<double> &xs);
     void init_inner(const ScaledProblem &sp);
     double power_sv();
@@ -618,6 +622,69 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     dx_ = dzeros<X>(n_);
     dy_ = dzeros<X>(m_);
     qx_ = dzeros<X>(n_);
+    tune_spmv();
+}
+
+// Mean duration (ms) of a launch sequence on the solver stream: one
+// untimed warm-up, then five timed repetitions.
+template <class V, class X> float Solver<V, X>::time_launches(const std::function<void()> &launch) {
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    launch();
+    HIP_OK(hipEventRecord(a, st_));
+    for (int r = 0; r < 5; ++r) launch();
+    HIP_OK(hipEventRecord(b, st_));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / 5.0f;
+}
+
+// SpMV autotuner: times the lane shifts -1, 0, +1 of the A and A' products
+// on the problem's own matrices (the shift moves every row class up or down
+// one power of two of lanes; the stored layout is the same) and keeps the
+// fastest, preferring the unshifted form unless another is 2% faster.  When
+// the column-panel form of a product was built, it is kept only if it beats
+// the best plain form (unless PDHCG_SPMV_PANELS=1 forces panels).
+// PDHCG_SPMV_SHIFT forces a shift.
+template <class V, class X> void Solver<V, X>::tune_spmv() {
+    const char *forced = std::getenv("PDHCG_SPMV_SHIFT");
+    const char *pe = std::getenv("PDHCG_SPMV_PANELS");
+    const bool panels_forced = pe && std::atoi(pe) == 1;
+    X *xs = dzeros<X>((size_t)std::max(n_, m_));
+    X *ys = dzeros<X>((size_t)std::max(n_, m_));
+    auto tune = [&](BinnedCsr<V> &M, PanelSpmv<V, Acc> &pn) {
+        int best = 0;
+        if (forced) best = std::atoi(forced);
+        else if (M.total_blocks > 0) {
+            float best_ms = time_launches([&] { spmv_launch<V, X, Acc>(M, xs, EpiStore<X>{ys}, st_); });
+            for (int sh : {-1, 1}) {
+                set_lane_shift(M, sh);
+                const float ms = time_launches([&] { spmv_launch<V, X, Acc>(M, xs, EpiStore<X>{ys}, st_); });
+                if (ms < 0.98f * best_ms) {
+                    best_ms = ms;
+                    best = sh;
+                }
+            }
+            if (pn.on && !forced && !panels_forced) {
+                const float pms = time_launches([&] { pn.launch(xs, EpiStore<X>{ys}, st_, nullptr); });
+                if (!(pms < best_ms)) pn.release();
+            }
+        }
+        set_lane_shift(M, best);
+        return best;
+    };
+    shift_A_ = tune(A_, pnA_);
+    shift_At_ = tune(At_, pnAt_);
+    HIP_OK(hipStreamSynchronize(st_));
+    (void)hipFree(xs);
+    (void)hipFree(ys);
+    if (P_.verbose >= 1)
+        std::fprintf(stderr, "[spmv] lane shift A %d%s, A' %d%s\n", shift_A_, pnA_.on ? " (column panels)" : "", shift_At_,
+                    pnAt_.on ? " (column panels)" : "");
 }
 
 // Initial auxiliary values of SOC / RSOC blocks written into the

@@ -90,6 +90,16 @@ def test_column_panel_spmv_matches_oracle(kind, monkeypatch):
         _check_pair(p, 1e-6)
 
 
+@pytest.mark.parametrize("shift", ["-1", "1"])
+def test_spmv_lane_shifts_match_oracle(shift, monkeypatch):
+    # the autotuner's candidate launch shapes (lanes per row halved / doubled
+    # for every row class), forced, on problems with short and long rows
+    monkeypatch.setenv("PDHCG_SPMV_SHIFT", shift)
+    for kind in ("lp", "lowrank_diag"):
+        _check_pair(workloads.random_problem(kind, seed=9, n=300, m=120), 1e-6)
+    _check_pair(workloads.config0(), 1e-4)
+
+
 def test_sparse_lowrank_csr_path():
     p = workloads.random_problem("lowrank_sparse_r", seed=3, n=1000, m=150)
     _check_pair(p, 1e-4)
