@@ -12,8 +12,11 @@ steps the same solver continues to the 1e-6 relative KKT tolerance, which
 gives the time-to-tolerance figure.
 
 Also reported: the achieved HBM bandwidth of the dominant kernel (HIP events
-on the solver stream; algorithmic bytes per launch / mean launch time), and
-the FP64 CPU oracle's iteration rate on a bounded sample of the same problem.
+on the solver stream; algorithmic bytes per launch / mean launch time), the
+FP64 CPU oracle's iteration rate on a bounded sample of the same problem, the
+batched portfolio workload, and the learned warm start (training data from
+the batched solver, device training, warm versus cold solves of unseen
+instances).
 """
 from __future__ import annotations
 
@@ -203,6 +206,87 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
     return out
 
 
+def mlp_step_flops(in_dim, hidden, out_dim, batch):
+    """Matrix-product flops of one training step (forward, weight gradients,
+    hidden-layer input gradients) at the padded sizes the device computes."""
+    pad = lambda v: (v + 15) // 16 * 16
+    i, h, o = pad(in_dim), hidden, pad(out_dim)
+    return 2.0 * batch * (2 * i * h + 3 * h * h + 3 * h * o)
+
+
+def run_warm(train_count, test_count, eps):
+    """The learned warm start on the parametric sparse QP + low-rank family
+    (n = 2000, m = 1000, nnz = 20000, rank 50, z in R^16): training solutions
+    from the batched solver at 1e-4, the 16 -> 1024 -> 1024 -> (n + m) network
+    trained on the device, then unseen instances solved cold and warm."""
+    import numpy as np
+    import pdhcg_amd
+    from pdhcg_amd import warmstart, workloads
+    fam = workloads.ParametricFamily()
+    Ztr, ptr = fam.sample(train_count, 11)
+    Zte, pte = fam.sample(test_count, 12)
+    t = time.perf_counter()
+    sols = pdhcg_amd.solve_batch(ptr, eps=1e-4, iteration_limit=100000, time_limit=300.0)
+    gen_s = time.perf_counter() - t
+    ok = [i for i, s_ in enumerate(sols) if s_ is not None and s_.status == "OPTIMAL"]
+    X = np.stack([sols[i].x for i in ok])
+    Y = np.stack([sols[i].y for i in ok])
+    del sols, ptr
+    log(f"[bench] warm: {len(ok)} of {train_count} training solutions in {gen_s:.1f}s")
+    epochs, batch = 300, 256
+    ws = warmstart.WarmStart(fam.z_dim, fam.n, fam.m, hidden=1024, seed=0)
+    t = time.perf_counter()
+    loss = ws.fit(Ztr[ok], X, Y, epochs=epochs, batch=batch, lr=1e-3)
+    train_s = time.perf_counter() - t
+    steps = epochs * (len(ok) // batch)
+    tflops = mlp_step_flops(fam.z_dim, 1024, fam.n + fam.m, batch) * steps / train_s / 1e12
+    log(f"[bench] warm: trained {steps} steps in {train_s:.1f}s ({tflops:.1f} TFLOP/s), loss {loss:.4f}")
+    t = time.perf_counter()
+    warm_p = [ws.start(p_, z) for p_, z in zip(pte, Zte)]
+    predict_s = time.perf_counter() - t
+    out = {"workload": "parametric sparse QP + low-rank family, n=%d m=%d nnz(A)=%d rank %d, z in R^%d; "
+                       "MLP %d->1024->1024->%d GELU, BF16 matrix cores / FP32 accumulate"
+                       % (fam.n, fam.m, fam.A.nnz, fam.R.shape[0], fam.z_dim, fam.z_dim, fam.n + fam.m),
+           "train_instances": len(ok), "test_instances": test_count, "train_solve_s": gen_s,
+           "train_s": train_s, "train_steps": steps, "train_loss": loss, "train_tflops": tflops,
+           "train_mfma_frac": tflops / 2500.0, "predict_s": predict_s}
+    for tol in (eps, 1e-4):
+        t = time.perf_counter()
+        cold = pdhcg_amd.solve_batch(pte, eps=tol, iteration_limit=100000, time_limit=300.0)
+        cold_s = time.perf_counter() - t
+        t = time.perf_counter()
+        warm = pdhcg_amd.solve_batch(warm_p, eps=tol, iteration_limit=100000, time_limit=300.0)
+        warm_s = time.perf_counter() - t
+        ic = np.array([s_.iterations for s_ in cold])
+        iw = np.array([s_.iterations for s_ in warm])
+        obj = max(abs(a.pobj - b.pobj) / (1 + abs(a.pobj)) for a, b in zip(cold, warm))
+        rec = {"cold_batch_s": cold_s, "warm_batch_s": warm_s, "cold_iterations_mean": float(ic.mean()),
+               "warm_iterations_mean": float(iw.mean()), "median_iteration_ratio": float(np.median(iw / ic)),
+               "optimal": [s_.status for s_ in warm].count("OPTIMAL"), "max_rel_objective_diff": obj}
+        if tol == eps:
+            xs = np.stack([s_.x for s_ in cold])
+            ys = np.stack([s_.y for s_ in cold])
+            xp = np.stack([p_.primal_start for p_ in warm_p])
+            yp = np.stack([p_.dual_start for p_ in warm_p])
+            out["prediction_rel_error"] = {"x": float(np.linalg.norm(xp - xs) / np.linalg.norm(xs)),
+                                           "y": float(np.linalg.norm(yp - ys) / np.linalg.norm(ys))}
+            sample = list(range(min(4, test_count)))
+            t = time.perf_counter()
+            for k in sample:
+                pdhcg_amd.solve(pte[k], eps=tol, iteration_limit=100000)
+            single_cold = (time.perf_counter() - t) / len(sample)
+            t = time.perf_counter()
+            for k in sample:
+                pdhcg_amd.solve(warm_p[k], eps=tol, iteration_limit=100000)
+            single_warm = (time.perf_counter() - t) / len(sample)
+            rec["single_device_cold_s"] = single_cold
+            rec["single_device_warm_s"] = single_warm
+        out["tol_%g" % tol] = rec
+        log(f"[bench] warm: tol {tol:g}: iterations cold {ic.mean():.0f} warm {iw.mean():.0f}")
+    ws.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +301,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=4096, metavar="N",
                     help="problems in the batched portfolio workload (0: skip it)")
+    ap.add_argument("--warm-train", type=int, default=4096, metavar="N",
+                    help="training instances of the learned warm start (0: skip it)")
+    ap.add_argument("--warm-test", type=int, default=256, metavar="N")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy used to calibrate rocprofv3 byte counters")
     args = ap.parse_args()
@@ -315,6 +402,8 @@ def main():
                                   "iterations": o["total_iters"], "rel_residuals": o["rel"], "pobj": o["pobj"]}
     if args.batch > 0:
         out["batched"] = run_batch(args.batch, args.eps, args.cpu_seconds, not args.no_cpu)
+    if args.warm_train > 0:
+        out["warm_start"] = run_warm(args.warm_train, args.warm_test, args.eps)
     print(json.dumps(out), flush=True)
 
 

@@ -167,3 +167,65 @@ def portfolio_batch(count: int, seed: int = 1003, n_range=(50, 2000), k_range=(1
     ks = rng.integers(k_range[0], k_range[1] + 1, size=count)
     return [portfolio_tracking(int(n), int(k), seed=(seed, i), te_factor=te_factor)
             for i, (n, k) in enumerate(zip(ns, ks))]
+
+
+class ParametricFamily:
+    """Parametric instances of the large sparse QP + low-rank family for the
+    learned warm start: one shared A (Zipf 1.5 row lengths), diagonal Q, R
+    (rank x n, N(0, 1/n)) and D; a parameter vector z ~ N(0, I) of z_dim
+    entries moves the objective and the row bounds:
+
+        c(z)  = c_bar + W_c z + N(0, 0.01^2)
+        x0(z) = clip(2.5 + W_x z, 0, 5)        (clipped before the bounds)
+        l, u  = A x0(z) -/+ s + N(0, 0.01^2),  s ~ U[0.1, 1] fixed
+
+    with W_c entries N(0, c_spread^2 / z_dim) and W_x entries
+    N(0, x_spread^2 / z_dim) (each coordinate of W z then has standard
+    deviation c_spread or x_spread; x_spread is kept small so that the row
+    bounds move less than their width s).  x0(z) lies in the box [0, 5]
+    and A x0(z) within [l, u] unless a noise draw exceeds s (at least ten
+    standard deviations), so every instance is feasible; `clipped` counts the
+    entries of x0 that the clip moved.
+    """
+
+    def __init__(self, n: int = 2000, m: int = 1000, nnz: int = 20000, rank: int = 50, z_dim: int = 16,
+                 seed: int = 1004, c_spread: float = 0.1, x_spread: float = 0.1):
+        rng = np.random.default_rng(seed)
+        self.n, self.m, self.z_dim = n, m, z_dim
+        lens = zipf_lengths(rng, m, nnz, 1.5, min(5000, n))
+        cols = _unique_columns_per_row(rng, lens, n)
+        rp = np.zeros(m + 1, dtype=np.int64)
+        np.cumsum(lens, out=rp[1:])
+        self.A = sp.csr_matrix((rng.standard_normal(int(rp[-1])), cols, rp.astype(np.int32)), shape=(m, n))
+        self.Q = sp.diags(rng.uniform(0.0, 1.0, n)).tocsr()
+        self.R = rng.standard_normal((rank, n)) * np.sqrt(1.0 / n)
+        self.D = rng.uniform(0.5, 2.0, rank)
+        self.c_bar = rng.standard_normal(n)
+        self.W_c = rng.standard_normal((n, z_dim)) * (c_spread / np.sqrt(z_dim))
+        self.W_x = rng.standard_normal((n, z_dim)) * (x_spread / np.sqrt(z_dim))
+        self.s = rng.uniform(0.1, 1.0, m)
+        self.clipped = 0
+
+    def instance(self, z, noise_seed) -> QPData:
+        rng = np.random.default_rng(noise_seed)
+        z = np.asarray(z, dtype=np.float64)
+        c = self.c_bar + self.W_c @ z + 0.01 * rng.standard_normal(self.n)
+        raw = 2.5 + self.W_x @ z
+        x0 = np.clip(raw, 0.0, 5.0)
+        self.clipped += int(np.count_nonzero(x0 != raw))
+        ax = self.A @ x0
+        lb = ax - self.s + 0.01 * rng.standard_normal(self.m)
+        ub = ax + self.s + 0.01 * rng.standard_normal(self.m)
+        return QPData(c=c, A=self.A, Q=self.Q, R=self.R, D=self.D, con_lb=lb, con_ub=ub, var_lb=np.zeros(self.n),
+                      var_ub=np.full(self.n, 5.0), fmt={"R": "dense"})
+
+    def sample(self, count: int, seed: int):
+        """(Z, problems): count parameter vectors and their instances;
+        instance i draws z and its noise from the seed pair (seed, i)."""
+        Z = np.empty((count, self.z_dim), dtype=np.float32)
+        problems = []
+        for i in range(count):
+            rng = np.random.default_rng((seed, i))
+            Z[i] = rng.standard_normal(self.z_dim)
+            problems.append(self.instance(Z[i], (seed, i, 1)))
+        return Z, problems
