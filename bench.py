@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--warm-train", type=int, default=4096, metavar="N",
                     help="training instances of the learned warm start (0: skip it)")
     ap.add_argument("--warm-test", type=int, default=256, metavar="N")
+    ap.add_argument("--warm-only", action="store_true",
+                    help="only run the learned warm start section (prints {\"warm_start\": ...})")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy used to calibrate rocprofv3 byte counters")
     args = ap.parse_args()
@@ -314,6 +316,10 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.warm_only:
+        print(json.dumps({"warm_start": run_warm(args.warm_train, args.warm_test, args.eps)}), flush=True)
+        return
 
     from pdhcg_amd import workloads
     t = time.time()
