@@ -54,6 +54,11 @@ def lib() -> C.CDLL:
         L.pdhcg_solver_timing.argtypes = [C.c_void_p, C.c_int]
         L.pdhcg_solver_kernel_time.restype = C.c_int
         L.pdhcg_solver_kernel_time.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_int)]
+        L.pdhcg_set_graph_window.restype = None
+        L.pdhcg_set_graph_window.argtypes = [C.c_int]
+        L.pdhcg_get_graph_window.restype = C.c_int
+        L.pdhcg_solver_graph_stats.restype = C.c_int
+        L.pdhcg_solver_graph_stats.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
         L.pdhcg_solve_batch.restype = C.c_int
         L.pdhcg_solve_batch.argtypes = [P(P(capi.qp_problem_t)), C.c_int, P(capi.pdhg_parameters_t),
                                         P(P(capi.pdhcg_result_t))]
@@ -71,6 +76,11 @@ def solve(p: QPData, precision: int = PRECISION_FP64, **params) -> Optional[capi
     L = lib()
     L.pdhcg_set_precision(precision)
     return _solve(L, "", p, default_params(L, "", **params))
+
+
+def set_graph_window(enable: bool) -> None:
+    """Graph replay of iteration windows for solvers created afterwards."""
+    lib().pdhcg_set_graph_window(1 if enable else 0)
 
 
 def solve_batch(problems, **params) -> list:
@@ -158,6 +168,12 @@ class Stepper:
         t, i = C.c_int(), C.c_int()
         self.L.pdhcg_solver_counts(self.s, C.byref(t), C.byref(i))
         return t.value, i.value
+
+    def graph_stats(self):
+        """(windows, iterations) replayed from recorded graphs so far."""
+        w, i = C.c_int(), C.c_int()
+        self.L.pdhcg_solver_graph_stats(self.s, C.byref(w), C.byref(i))
+        return w.value, i.value
 
     def result(self) -> capi.Solution:
         r = self.L.pdhcg_solver_result(self.s)

@@ -38,7 +38,18 @@ template <class X> struct EpiDualStep {
     const X *g, *cl, *cu, *y0;
     double sigma, w, rho;
     int store_pdhg, halpern;
+    // When set, sigma and w come from device memory: dyn = (tau, sigma, k0)
+    // with w = (k0 + pos + 1) / (k0 + pos + 2), so that one recorded launch
+    // serves every window of iterations (graph replay).
+    const double *dyn = nullptr;
+    int pos = 0;
     template <class Acc> __device__ void operator()(int r, Acc s) const {
+        double sigma = this->sigma, w = this->w;
+        if (dyn) {
+            sigma = dyn[1];
+            const double k = dyn[2] + pos;
+            w = (k + 1.0) / (k + 2.0);
+        }
         const double cur = (double)y[r];
         const double v = (double)s + (double)g[r];
         const double pj = fmax((double)cl[r], fmin(v - cur / sigma, (double)cu[r]));
@@ -61,7 +72,15 @@ template <class X> struct EpiPrimalStep {
     const X *c, *lb, *ub, *x0, *qdiag;  // qdiag null for LP
     double tau, w, rho;
     int store_pdhg, halpern;
+    const double *dyn = nullptr;  // as in EpiDualStep: tau = dyn[0]
+    int pos = 0;
     template <class Acc> __device__ void operator()(int r, Acc s) const {
+        double tau = this->tau, w = this->w;
+        if (dyn) {
+            tau = dyn[0];
+            const double k = dyn[2] + pos;
+            w = (k + 1.0) / (k + 2.0);
+        }
         const double xi = (double)x[r];
         double t = xi - tau * ((double)c[r] - (double)s);
         if (qdiag) t = t / (1.0 + tau * (double)qdiag[r]);

@@ -45,6 +45,30 @@ int print_frequency(int iter) {  // reference src/utils.cu:535-546
     return step;
 }
 
+// Prologue of a recorded window of iterations: writes the window's scalars
+// (primal step, dual step, Halpern count at the window start) to device
+// memory, where the step epilogues of the window read them.
+__global__ void graph_dyn_kernel(double *dyn, double tau, double sigma, double k0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dyn[0] = tau;
+        dyn[1] = sigma;
+        dyn[2] = k0;
+    }
+}
+
+// Graph replay is off by default: measured on MI355X, the launch-by-launch
+// loop already runs at the device's dispatch rate (about 6.5 us per SpMV
+// launch, the host running ahead), and a replayed window was 3-7% slower.
+int g_graph_window = -1;  // -1: read PDHCG_GRAPH once
+
+bool graph_window_default() {
+    if (g_graph_window < 0) {
+        const char *e = std::getenv("PDHCG_GRAPH");
+        g_graph_window = (e && std::strcmp(e, "1") == 0) ? 1 : 0;
+    }
+    return g_graph_window != 0;
+}
+
 template <class T> T *dalloc(size_t count) {
     T *p = nullptr;
     HIP_OK(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
@@ -311,6 +335,7 @@ class SolverBase {
     virtual void synchronize() = 0;
     virtual double setup_seconds() const = 0;
     virtual KernelTimer &timer() = 0;
+    virtual void graph_stats(int *windows, int *iterations) const = 0;
 };
 
 template <class V, class X> class Solver final : public SolverBase {
@@ -327,6 +352,10 @@ template <class V, class X> class Solver final : public SolverBase {
     void synchronize() override { HIP_OK(hipStreamSynchronize(st_)); }
     double setup_seconds() const override { return setup_seconds_; }
     KernelTimer &timer() override { return timer_; }
+    void graph_stats(int *windows, int *iterations) const override {
+        *windows = graph_windows_;
+        *iterations = graph_iters_;
+    }
 
   private:
     using Acc = typename std::conditional<std::is_same<V, double>::value || std::is_same<X, double>::value, double,
@@ -439,6 +468,32 @@ template <class V, class X> class Solver final : public SolverBase {
     hipEvent_t bb_ev_[2] = {nullptr, nullptr};
     int bb_pred_ = 2;          // BB bodies the previous inner solve ran
 
+    // Recorded windows of plain iterations (closed-form prox, no cones): runs
+    // of iterations with no evaluation, restart or fixed-point error are
+    // replayed from an instantiated hipGraph holding their SpMV launches.
+    // One graph per (length, store pattern); the window's scalars are set by
+    // the graph's prologue node before each replay.
+    struct GraphWindow {
+        int len = 0;
+        std::vector<char> store;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipGraphNode_t prologue = nullptr;
+        hipEvent_t done = nullptr;
+        long long used = 0;
+    };
+    static constexpr int kMinGraphWindow = 4, kMaxGraphWindow = 256, kMaxGraphs = 8;
+    std::vector<GraphWindow> graphs_;
+    double *dyn_ = nullptr;
+    bool graph_on_ = false;
+    int graph_windows_ = 0, graph_iters_ = 0;
+    long long graph_clock_ = 0;
+    bool plain_iteration(int t) const;
+    double primal_tau() const;
+    GraphWindow &graph_for(int len, const std::vector<char> &store);
+    void run_window(int len);
+    void release_graph(GraphWindow &g);
+
     // outer-loop state (reference internal/internal_types.h:137-236)
     double q_norm_ = 0, nonconvexity_ = 0;
     double step_ = 0, pw_ = 1, pw_err_sum_ = 0, pw_last_err_ = 0, best_pw_ = 1, best_gap_ = INFINITY;
@@ -498,6 +553,8 @@ template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const 
     }
     pw_ = P_.bound_objective_rescaling ? 1.0 : (obj_norm_ + 1.0) / (bound_norm_ + 1.0);
     best_pw_ = pw_;
+    graph_on_ = graph_window_default() && closed_prox() && !has_cones_ && !has_affine_;
+    if (graph_on_) dyn_ = dzeros<double>(4);
     HIP_OK(hipStreamSynchronize(st_));
     t_start_ = std::chrono::steady_clock::now();
     setup_seconds_ = std::chrono::duration<double>(t_start_ - t0).count();
@@ -765,6 +822,8 @@ template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &s
 
 template <class V, class X> Solver<V, X>::~Solver() {
     if (st_) (void)hipStreamSynchronize(st_);
+    for (GraphWindow &g : graphs_) release_graph(g);
+    if (dyn_) (void)hipFree(dyn_);
     free_binned(A_);
     free_binned(At_);
     free_binned(Q_);
@@ -1022,12 +1081,116 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
 }
 
 // One PDHG step (reference src/pdhg_core_op.cu:705-830).
-template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
+template <class V, class X> double Solver<V, X>::primal_tau() const {
     double tau = step_ / pw_;
     if (nonconvexity_ < 0) {
         tau = std::fmax(tau, -1.01 * std::fmin(0.0, nonconvexity_));
         tau /= 100;
     }
+    return tau;
+}
+
+// Iteration t runs no evaluation, restart or fixed-point error, given that the
+// previous one did not end a major cycle (run() below).
+template <class V, class X> bool Solver<V, X>::plain_iteration(int t) const {
+    const int ef = P_.termination_evaluation_frequency;
+    return t > 0 && t % ef != 0 && t % print_frequency(t) != 0 && (t + 1) % ef != 0;
+}
+
+template <class V, class X> void Solver<V, X>::release_graph(GraphWindow &g) {
+    if (g.done) (void)hipEventSynchronize(g.done);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.done) (void)hipEventDestroy(g.done);
+    g = GraphWindow();
+}
+
+// The instantiated graph of `len` plain iterations with the given store
+// pattern: a prologue kernel node, then the iterations' launches exactly as
+// pdhg_step enqueues them (fused Halpern), recorded by stream capture.
+template <class V, class X>
+typename Solver<V, X>::GraphWindow &Solver<V, X>::graph_for(int len, const std::vector<char> &store) {
+    for (GraphWindow &g : graphs_)
+        if (g.len == len && g.store == store) {
+            g.used = ++graph_clock_;
+            return g;
+        }
+    if ((int)graphs_.size() >= kMaxGraphs) {
+        auto lru = std::min_element(graphs_.begin(), graphs_.end(),
+                                    [](const GraphWindow &a, const GraphWindow &b) { return a.used < b.used; });
+        release_graph(*lru);
+        graphs_.erase(lru);
+    }
+    const double rho = P_.reflection_coefficient;
+    hipGraph_t body = nullptr;
+    HIP_OK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int p = 0; p < len; ++p) {
+            EpiPrimalStep<X> epi{x_, xp_, xr_, dslack_, c_, lb_, ub_, x0_, qtype_ == PDHCG_DIAG_Q ? qdiag_ : nullptr,
+                                 0.0, 0.0, rho, (int)store[p], 1, dyn_, p};
+            apply_At(y_, epi, st_);
+            EpiDualStep<X> depi{y_, yp_, yr_, g_, cl_, cu_, y0_, 0.0, 0.0, rho, (int)store[p], 1, dyn_, p};
+            apply_A(xr_, depi, st_);
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(st_, &body);
+        if (body) (void)hipGraphDestroy(body);
+        throw;
+    }
+    HIP_OK(hipStreamEndCapture(st_, &body));
+    GraphWindow g;
+    g.len = len;
+    g.store = store;
+    HIP_OK(hipGraphCreate(&g.graph, 0));
+    double *d = dyn_, zero = 0.0;
+    void *args[] = {&d, &zero, &zero, &zero};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void *>(graph_dyn_kernel);
+    kp.gridDim = dim3(1);
+    kp.blockDim = dim3(kWave);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    HIP_OK(hipGraphAddKernelNode(&g.prologue, g.graph, nullptr, 0, &kp));
+    hipGraphNode_t child = nullptr;
+    HIP_OK(hipGraphAddChildGraphNode(&child, g.graph, &g.prologue, 1, body));
+    HIP_OK(hipGraphDestroy(body));
+    HIP_OK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    HIP_OK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    g.used = ++graph_clock_;
+    graphs_.push_back(g);
+    return graphs_.back();
+}
+
+// Replays `len` plain iterations from total_count_ (state as after len
+// passes of run()'s loop body).
+template <class V, class X> void Solver<V, X>::run_window(int len) {
+    const int tc = total_count_;
+    std::vector<char> store((size_t)len);
+    for (int p = 0; p < len; ++p) store[p] = ((tc + p + 2) % print_frequency(tc + p + 2)) == 0 ? 1 : 0;
+    GraphWindow &g = graph_for(len, store);
+    double *d = dyn_, tau = primal_tau(), sigma = step_ * pw_, k0 = (double)inner_count_;
+    void *args[] = {&d, &tau, &sigma, &k0};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void *>(graph_dyn_kernel);
+    kp.gridDim = dim3(1);
+    kp.blockDim = dim3(kWave);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    HIP_OK(hipEventSynchronize(g.done));  // the previous replay of this graph has finished
+    HIP_OK(hipGraphExecKernelNodeSetParams(g.exec, g.prologue, &kp));
+    HIP_OK(hipGraphLaunch(g.exec, st_));
+    HIP_OK(hipEventRecord(g.done, st_));
+    host_inner_ += len;
+    inner_count_ += len;
+    total_count_ += len;
+    graph_windows_ += 1;
+    graph_iters_ += len;
+}
+
+template <class V, class X> void Solver<V, X>::pdhg_step(bool fuse_halpern) {
+    double tau = primal_tau();
     const double sigma = step_ * pw_;
     const int tc = total_count_;
     const int store = (is_major_ || ((tc + 2) % print_frequency(tc + 2)) == 0) ? 1 : 0;
@@ -1321,6 +1484,16 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
             residual();
             break;
         }
+        if (graph_on_ && !timer_.enabled && !is_major_ && !do_restart_ && plain_iteration(total_count_)) {
+            const int lim = std::min(iterations - done, P_.termination_criteria.iteration_limit - total_count_);
+            int len = 1;
+            while (len < lim && len < kMaxGraphWindow && plain_iteration(total_count_ + len)) ++len;
+            if (len >= kMinGraphWindow) {
+                run_window(len);
+                done += len - 1;
+                continue;
+            }
+        }
         const int tc = total_count_;
         if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
             residual();
@@ -1538,6 +1711,16 @@ int pdhcg_solver_timing(pdhcg_solver_t *s, int enable) {
         std::fprintf(stderr, "[interface] %s\n", e.what());
         return -1;
     }
+}
+
+void pdhcg_set_graph_window(int enable) { g_graph_window = enable ? 1 : 0; }
+
+int pdhcg_get_graph_window(void) { return graph_window_default() ? 1 : 0; }
+
+int pdhcg_solver_graph_stats(pdhcg_solver_t *s, int *windows, int *iterations) {
+    if (!s || !windows || !iterations) return -1;
+    s->impl->graph_stats(windows, iterations);
+    return 0;
 }
 
 int pdhcg_solver_kernel_time(pdhcg_solver_t *s, int kernel, double *total_ms, int *launches) {
