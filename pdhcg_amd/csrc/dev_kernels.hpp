@@ -395,6 +395,20 @@ inline int rx_chunks(int n) { return (n + kRxWaveCols - 1) / kRxWaveCols; }
 template <class V> __device__ __forceinline__ double stream_load(const V *p) {
     return (double)__builtin_nontemporal_load(p);
 }
+template <class V> struct Vec2;
+template <> struct Vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <> struct Vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+// Two adjacent elements; p must be aligned to 2 elements.
+template <class V> __device__ __forceinline__ void stream_load2(const V *p, double (&o)[2]) {
+    typedef typename Vec2<V>::type T;
+    const T t = __builtin_nontemporal_load(reinterpret_cast<const T *>(p));
+    o[0] = (double)t[0];
+    o[1] = (double)t[1];
+}
 
 // Dense R storage: column tiles of kRxWaveCols columns x k rows, each tile
 // row-major and contiguous, the last tile zero padded.  One wave's share of
@@ -420,15 +434,25 @@ __device__ __forceinline__ double wave_sum4(const double (&s)[4], int lane) {
     return u;
 }
 
-// RU rows of a tile over this lane's kRxLaneCols columns: all RU *
-// kRxLaneCols loads are issued before the multiply-adds (one register block).
+// Column of R x handled by lane `lane` in slot q: two adjacent columns per
+// 16-byte load, pairs 128 columns apart (2 lane + 128 (q / 2) + q % 2).
+__device__ __forceinline__ int rx_lane_col(int lane, int q) { return 2 * lane + (q / 2) * (kRxWaveCols / 2) + (q % 2); }
+static_assert(kRxLaneCols == 4, "two column pairs per lane");
+
+// RU rows of a tile over this lane's kRxLaneCols columns: all RU * 2 pair
+// loads are issued before the multiply-adds (one register block).
 template <int RU, class V>
 __device__ __forceinline__ void rx_rows(const V *T, int j0, int lane, const double (&xv)[kRxLaneCols], double (&s)[RU]) {
     double r[RU][kRxLaneCols];
 #pragma unroll
     for (int a = 0; a < RU; ++a)
 #pragma unroll
-        for (int q = 0; q < kRxLaneCols; ++q) r[a][q] = stream_load(T + (size_t)(j0 + a) * kRxWaveCols + lane + q * kWave);
+        for (int h = 0; h < 2; ++h) {
+            double pr[2];
+            stream_load2(T + (size_t)(j0 + a) * kRxWaveCols + rx_lane_col(lane, 2 * h), pr);
+            r[a][2 * h] = pr[0];
+            r[a][2 * h + 1] = pr[1];
+        }
 #pragma unroll
     for (int a = 0; a < RU; ++a) {
         double t = 0.0;
@@ -450,12 +474,12 @@ __global__ void __launch_bounds__(kBlock) rx_partial_kernel(const V *R, const X 
     const int lane = threadIdx.x % kWave;
     const int chunk = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (chunk >= nchunks) return;
-    const long long base = (long long)chunk * kRxWaveCols + lane;
+    const long long base = (long long)chunk * kRxWaveCols;
     const V *T = R + (size_t)chunk * k * kRxWaveCols;
     double xv[kRxLaneCols];
 #pragma unroll
     for (int q = 0; q < kRxLaneCols; ++q) {
-        const long long c = base + q * kWave;
+        const long long c = base + rx_lane_col(lane, q);
         xv[q] = c < n ? (double)x[c] : 0.0;
     }
     auto emit = [&](const double (&s4)[4], int j0) {
@@ -529,7 +553,39 @@ __global__ void __launch_bounds__(kBlock) rx_fold_kernel(const double *partials,
     if (threadIdx.x == 0) *counter = 0u;
 }
 
-// qx = R'u (+ qx, or + qdiag .* x)  with R dense in column tiles, one thread per column.
+// Two adjacent columns i0, i0 + 1 of R'u (one 16-byte load per row of R in
+// FP64), each column's sum over the rows in order j = 0 .. k - 1.  Columns
+// past n lie in the zero padding of the last tile.  On MI355X this pattern
+// streams the tiles at about 6.4 TB/s against 6.2 TB/s for one column per
+// lane; four columns per lane need too many registers for ten rows in flight
+// (4.0 TB/s).
+constexpr int kRtuCols = 2;
+template <class V>
+__device__ __forceinline__ void rtu_cols(const V *R, const double *u, long long i0, int k, double (&s)[kRtuCols]) {
+    constexpr int JU = 10;  // rows of R loaded before the multiply-adds
+    const V *col = R + rx_tile_index(0, i0, k);  // row j of these columns at col[j * kRxWaveCols]
+#pragma unroll
+    for (int q = 0; q < kRtuCols; ++q) s[q] = 0.0;
+    int j = 0;
+    for (; j + JU <= k; j += JU) {
+        double r[JU][kRtuCols];
+#pragma unroll
+        for (int a = 0; a < JU; ++a) stream_load2(col + (size_t)(j + a) * kRxWaveCols, r[a]);
+#pragma unroll
+        for (int a = 0; a < JU; ++a)
+#pragma unroll
+            for (int q = 0; q < kRtuCols; ++q) s[q] += r[a][q] * u[j + a];
+    }
+    for (; j < k; ++j) {
+        double r[kRtuCols];
+        stream_load2(col + (size_t)j * kRxWaveCols, r);
+#pragma unroll
+        for (int q = 0; q < kRtuCols; ++q) s[q] += r[q] * u[j];
+    }
+}
+
+// qx = R'u (+ qx, or + qdiag .* x)  with R dense in column tiles, kRtuCols
+// adjacent columns per thread (grid over ceil(n / kRtuCols) threads).
 // Gated launches (inside the BB loop) are a separate instantiation so that
 // profiles separate them from the always-running ones.
 template <class V, class X, bool Gated>
@@ -537,57 +593,56 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
                                                             const X *qdiag, const X *x, int accumulate,
                                                             const int *gate) {
     if (Gated && *gate == 0) return;
-    constexpr int JU = 10;  // rows of R loaded before the multiply-adds
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const V *col = R + rx_tile_index(0, i, k);  // row j of this column at col[j * kRxWaveCols]
-        double s = 0.0;
-        int j = 0;
-        for (; j + JU <= k; j += JU) {
-            double r[JU];
+    const long long groups = ((long long)n + kRtuCols - 1) / kRtuCols;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long i0 = kRtuCols * t;
+        double s[kRtuCols];
+        rtu_cols(R, u, i0, k, s);
 #pragma unroll
-            for (int a = 0; a < JU; ++a) r[a] = stream_load(col + (size_t)(j + a) * kRxWaveCols);
-#pragma unroll
-            for (int a = 0; a < JU; ++a) s += r[a] * u[j + a];
+        for (int q = 0; q < kRtuCols; ++q) {
+            const long long i = i0 + q;
+            if (i >= n) break;
+            double v = s[q];
+            if (qdiag) v = v + (double)qdiag[i] * (double)x[i];
+            else if (accumulate) v = v + (double)qx[i];
+            qx[i] = (X)v;
         }
-        for (; j < k; ++j) s += stream_load(col + (size_t)j * kRxWaveCols) * u[j];
-        if (qdiag) s = s + (double)qdiag[i] * (double)x[i];
-        else if (accumulate) s = s + (double)qx[i];
-        qx[i] = (X)s;
     }
 }
 
 // BB body on the dense low-rank path: the R'u pass fused with the gradient
 // step of bb_gradient_kernel.  Qx_p = R'u (+ diag(Q) x_p, or + the sparse-Q
 // product already in qx_sparse); g = Qx_p + c - A'y + (x_p - x)/tau;
-// partial of s'(g - g_old); grad = g.  Qx_p itself is not stored.
+// partial of s'(g - g_old); grad = g.  Qx_p itself is not stored.  kRtuCols
+// adjacent columns per thread, as rtu_dense_kernel.
 template <class V, class X>
 __global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const double *u, int n, int k,
                                                            const X *qx_sparse, const X *qdiag, const X *xp, const X *x,
                                                            const X *c, const X *aty, X *grad, const X *dirn,
                                                            double inv_tau, GridRed g, const DevScalars *ds) {
     if (ds->bb_active == 0) return;
-    constexpr int JU = 10;
     double v[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const V *col = R + rx_tile_index(0, i, k);
-        double s = 0.0;
-        int j = 0;
-        for (; j + JU <= k; j += JU) {
-            double r[JU];
+    const long long groups = ((long long)n + kRtuCols - 1) / kRtuCols;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long i0 = kRtuCols * t;
+        double s[kRtuCols];
+        rtu_cols(R, u, i0, k, s);
 #pragma unroll
-            for (int a = 0; a < JU; ++a) r[a] = stream_load(col + (size_t)(j + a) * kRxWaveCols);
-#pragma unroll
-            for (int a = 0; a < JU; ++a) s += r[a] * u[j + a];
+        for (int q = 0; q < kRtuCols; ++q) {
+            const long long i = i0 + q;
+            if (i >= n) break;
+            double sq = s[q];
+            const double xpi = (double)xp[i];
+            if (qdiag) sq = sq + (double)qdiag[i] * xpi;
+            else if (qx_sparse) sq = sq + (double)qx_sparse[i];
+            const double qv = (double)(X)sq;  // the stored product's rounding, as bb_gradient_kernel sees it
+            const double cur = qv + (double)c[i] - (double)aty[i] + inv_tau * (xpi - (double)x[i]);
+            const double dg = (double)(X)(cur - (double)grad[i]);
+            grad[i] = (X)cur;
+            v[0] += (double)dirn[i] * dg;
         }
-        for (; j < k; ++j) s += stream_load(col + (size_t)j * kRxWaveCols) * u[j];
-        const double xpi = (double)xp[i];
-        if (qdiag) s = s + (double)qdiag[i] * xpi;
-        else if (qx_sparse) s = s + (double)qx_sparse[i];
-        const double q = (double)(X)s;  // the stored product's rounding, as bb_gradient_kernel sees it
-        const double cur = q + (double)c[i] - (double)aty[i] + inv_tau * (xpi - (double)x[i]);
-        const double dg = (double)(X)(cur - (double)grad[i]);
-        grad[i] = (X)cur;
-        v[0] += (double)dirn[i] * dg;
     }
     grid_partials<1, 0>(g, v);
 }
