@@ -440,6 +440,274 @@ __global__ void __launch_bounds__(kBlock) cone_dual_residual_kernel(double *dres
     }
 }
 
+// ---------------- long SOC / RSOC blocks (one workgroup per block) ----------------
+//
+// Blocks with more than kMaxThreadCone vector slots (the reference's warp,
+// block and grid methods, src/kernels/pdhcg_soc_cone_kernels.cu:727-810,
+// 910-1148, 1725-2443) run the same algorithms as soc_project_block and
+// soc_dual_residual_block with one workgroup per block: every sum over the
+// block, including each evaluation of the multiplier function inside the
+// bisection, is a fixed-order workgroup reduction broadcast to all threads,
+// so all threads take the same branches.  The slots stay in global memory.
+
+// Sums NV per-thread values over the workgroup; every thread receives the totals.
+template <int NV> struct WgSum {
+    double lds[NV * 16];
+    double bc[NV];
+    __device__ void operator()(double (&v)[NV]) {
+        block_reduce<NV, 0>(v, lds);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) bc[i] = v[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = bc[i];
+        __syncthreads();
+    }
+};
+
+// soc_multiplier with f summed over the workgroup; p(m), h(m) give slot m's
+// value and relative scaling.  Called by every thread of the workgroup.
+template <class Pf, class Hf>
+__device__ double soc_multiplier_wg(Pf p, Hf h, int k, double w, double dw, double z, double *warm, WgSum<1> &red) {
+    auto f = [&](double lam) {
+        double v[1] = {0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double hm = h(m);
+            const double t = p(m) * hm / (hm * hm + 2.0 * lam);
+            v[0] += t * t;
+        }
+        red(v);
+        const double tw = w * dw / (dw * dw + 2.0 * lam);
+        const double zt = z / (1.0 - 2.0 * lam);
+        return v[0] + tw * tw - zt * zt;
+    };
+    double lo, hi;
+    const bool zpos = z > 0.0;
+    if (zpos) {
+        lo = 0.0;
+        hi = 0.5 - 1e-14;
+    } else {
+        lo = 0.5 + 1e-14;
+        hi = 1.0;
+        for (int it = 0; it < 60; ++it) {
+            if (f(hi) > 0.0) break;
+            lo = hi;
+            hi *= 2.0;
+        }
+    }
+    const double wl = *warm;
+    if (wl > lo && wl < hi) {
+        const double fv = f(wl);
+        if (fabs(fv) < 1e-12) return wl;
+        if (zpos == (fv > 0.0)) lo = wl;
+        else hi = wl;
+    }
+    for (int it = 0; it < 60; ++it) {
+        const double lam = 0.5 * (lo + hi);
+        const double fv = f(lam);
+        if (zpos == (fv > 0.0)) lo = lam;
+        else hi = lam;
+        if ((hi - lo) / (1.0 + hi + lo) < 1e-13) break;
+    }
+    const double lam = 0.5 * (lo + hi);
+    __syncthreads();  // every thread has read *warm
+    if (threadIdx.x == 0) *warm = lam;
+    return lam;
+}
+
+// Projection of long blocks onto D K (q == null) or, for the diagonal-Q prox,
+// onto D K in the metric diag(w), w = 1 + tau q, as W^-1/2 P_{W^1/2 D K}(W^1/2 v)
+// (cone_project_kernel / cone_project_diag_q_kernel).  Grid: L.count workgroups.
+template <class X, bool Rotated>
+__global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const double *d, const X *q, double tau,
+                                                                   ConeList L, int use_rwarm, const int *gate) {
+    if (gate && *gate == 0) return;
+    __shared__ WgSum<2> red2;
+    __shared__ WgSum<1> red1;
+    const int b = blockIdx.x;
+    const int s = L.start[b], k = L.vdim[b];
+    double *warm = use_rwarm ? L.rwarm + b : L.warm + b;
+    auto sw = [&](int m) {
+        if (!q) return 1.0;
+        const double w = 1.0 + tau * (double)q[s + m];
+        return sqrt(w > 0.0 ? w : 1.0);
+    };
+    auto P = [&](int m) { return (double)v[s + m] * sw(m); };
+    auto E = [&](int m) { return d[s + m] * sw(m); };
+    const double a = P(k), bb = P(k + 1);
+    const double w = Rotated ? (a - bb) * kInvSqrt2 : a;
+    const double z = Rotated ? (a + bb) * kInvSqrt2 : bb;
+    const double dref = Rotated ? sqrt(E(k) * E(k + 1)) : E(k + 1);
+    const double dw = Rotated ? 1.0 : E(k) / dref;
+    bool uniform = Rotated || dw == 1.0;
+    {
+        double nu[2] = {0.0, 0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x)
+            if (E(m) != dref) nu[0] = 1.0;
+        red2(nu);
+        uniform = uniform && nu[0] == 0.0;
+    }
+    // final transform: mode 0 keep, 1 zero, 2 uniform scale sc, 3 weighted with lam
+    int mode;
+    double sc = 0.0, lam = 0.0, wn = 0.0, zn = 0.0;
+    if (uniform) {
+        double ss[2] = {0.0, 0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double pm = P(m);
+            ss[0] += pm * pm;
+        }
+        red2(ss);
+        const double r = sqrt(w * w + ss[0]);
+        if (r <= z) mode = 0;
+        else if (r <= -z) mode = 1;
+        else {
+            mode = 2;
+            sc = (z + r) / (2.0 * r);
+            wn = sc * w;
+            zn = sc * r;
+        }
+    } else {
+        auto h = [&](int m) { return E(m) / dref; };
+        double rr[2] = {0.0, 0.0};  // sum (p/h)^2, sum (p h)^2 over the vector slots
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double hm = h(m), pm = P(m);
+            rr[0] += (pm / hm) * (pm / hm);
+            rr[1] += (pm * hm) * (pm * hm);
+        }
+        red2(rr);
+        const double ri = (w / dw) * (w / dw) + rr[0], rp = (w * dw) * (w * dw) + rr[1];
+        if (sqrt(ri) <= z) mode = 0;
+        else if (sqrt(rp) <= -z) mode = 1;
+        else {
+            mode = 3;
+            lam = soc_multiplier_wg(P, h, k, w, dw, z, warm, red1);
+            wn = Rotated ? w / (1.0 + 2.0 * lam) : w * dw * dw / (dw * dw + 2.0 * lam);
+            zn = z / (1.0 - 2.0 * lam);
+        }
+    }
+    if (mode == 0 && !q) return;
+    // every slot is read above before any is written: the reductions end with a barrier
+    for (int m = threadIdx.x; m < k + 2; m += blockDim.x) {
+        double pm = P(m);
+        if (mode == 1) pm = 0.0;
+        else if (mode >= 2 && m < k) {
+            if (mode == 2) pm *= sc;
+            else {
+                const double hm = E(m) / dref;
+                pm = pm * hm * hm / (hm * hm + 2.0 * lam);
+            }
+        } else if (mode >= 2) {
+            if (Rotated) pm = m == k ? (zn + wn) * kInvSqrt2 : (zn - wn) * kInvSqrt2;
+            else pm = m == k ? wn : zn;
+        }
+        v[s + m] = (X)(pm / sw(m));
+    }
+}
+
+// Dual residual of long blocks (cone_dual_residual_kernel): (rc - P_{K*}(rc)) d
+// with rc = c + qx - aty.  Grid: L.count workgroups.
+template <class X, bool Rotated>
+__global__ void __launch_bounds__(kBlock) soc_long_dual_residual_kernel(double *dres, const X *c, const X *qx,
+                                                                         const X *aty, const double *d, ConeList L) {
+    __shared__ WgSum<2> red2;
+    __shared__ WgSum<1> red1;
+    const int b = blockIdx.x;
+    const int s = L.start[b], k = L.vdim[b];
+    auto rc = [&](int m) {
+        const int i = s + m;
+        return (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i];
+    };
+    auto D = [&](int m) { return d[s + m]; };
+    const double a = rc(k), bb = rc(k + 1);
+    const double rw = Rotated ? (a - bb) * kInvSqrt2 : a;
+    const double rz = Rotated ? (a + bb) * kInvSqrt2 : bb;
+    const double dref = Rotated ? sqrt(D(k) * D(k + 1)) : D(k + 1);
+    const double ew = Rotated ? 1.0 : dref / D(k);
+    bool uniform = Rotated || ew == 1.0;
+    {
+        double nu[2] = {0.0, 0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x)
+            if (D(m) != dref) nu[0] = 1.0;
+        red2(nu);
+        uniform = uniform && nu[0] == 0.0;
+    }
+    double pa, pb;
+    if (uniform) {
+        double ss[2] = {0.0, 0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double pm = rc(m);
+            ss[0] += pm * pm;
+        }
+        red2(ss);
+        const double r = sqrt(rw * rw + ss[0]);
+        double vf, pw, pz;
+        if (r <= rz) {
+            vf = 0.0;
+            pw = rw;
+            pz = rz;
+        } else if (r <= -rz) {
+            vf = 1.0;
+            pw = 0.0;
+            pz = 0.0;
+        } else {
+            const double sc = (rz + r) / (2.0 * r);
+            vf = 1.0 - sc;
+            pw = sc * rw;
+            pz = sc * r;
+        }
+        for (int m = threadIdx.x; m < k; m += blockDim.x) dres[s + m] = rc(m) * vf * D(m);
+        if (Rotated) {
+            if (r <= rz) {
+                pa = a;
+                pb = bb;
+            } else {
+                pa = (pz + pw) * kInvSqrt2;
+                pb = (pz - pw) * kInvSqrt2;
+            }
+        } else {
+            pa = pw;
+            pb = pz;
+        }
+    } else {
+        auto h = [&](int m) { return dref / D(m); };
+        double rr[2] = {0.0, 0.0};
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double hm = h(m), pm = rc(m);
+            rr[0] += (pm / hm) * (pm / hm);
+            rr[1] += (pm * hm) * (pm * hm);
+        }
+        red2(rr);
+        const double ri = (rw / ew) * (rw / ew) + rr[0], rp = (rw * ew) * (rw * ew) + rr[1];
+        if (sqrt(ri) <= rz) {
+            for (int m = threadIdx.x; m < k + 2; m += blockDim.x) dres[s + m] = 0.0;
+            return;
+        }
+        if (sqrt(rp) <= -rz) {
+            for (int m = threadIdx.x; m < k + 2; m += blockDim.x) dres[s + m] = rc(m) * D(m);
+            return;
+        }
+        const double lam = soc_multiplier_wg(rc, h, k, rw, ew, rz, L.rwarm + b, red1);
+        for (int m = threadIdx.x; m < k; m += blockDim.x) {
+            const double hm = h(m), pm = rc(m);
+            dres[s + m] = (pm - pm * hm * hm / (hm * hm + 2.0 * lam)) * D(m);
+        }
+        const double zn = rz / (1.0 - 2.0 * lam);
+        const double wn = Rotated ? rw / (1.0 + 2.0 * lam) : rw * ew * ew / (ew * ew + 2.0 * lam);
+        if (Rotated) {
+            pa = (zn + wn) * kInvSqrt2;
+            pb = (zn - wn) * kInvSqrt2;
+        } else {
+            pa = wn;
+            pb = zn;
+        }
+    }
+    if (threadIdx.x == 0) {
+        dres[s + k] = (a - pa) * D(k);
+        dres[s + k + 1] = (bb - pb) * D(k + 1);
+    }
+}
+
 // Power-cone primal violation (reference pdhcg_power_cone_kernels.cu:621-669),
 // reduced over blocks: out = [sum abs^2, sum rel^2, max abs, max rel].
 template <class X>

@@ -113,6 +113,7 @@ class DevConeSet {
   public:
     bool psd = false, power = false;
     ConeList rsoc, soc, expc, powc;
+    ConeList lrsoc, lsoc;                        // SOC/RSOC blocks above kMaxThreadCone vector slots
     ConeList powall;                             // every power block, for the primal violation
     SectionList sec;                             // blocks with fixed slots (any non-PSD type)
     char *fixed = nullptr;                       // per-slot fixed flags, when sec.count > 0
@@ -130,7 +131,7 @@ class DevConeSet {
         struct HostList {
             std::vector<int> start, vdim;
             std::vector<double> alpha;
-        } lists[4];  // rotated SOC, SOC, exponential, power
+        } lists[6];  // rotated SOC, SOC, exponential, power, long rotated SOC, long SOC
         std::vector<char> hmask((size_t)n, 0);
         std::vector<std::pair<int, int>> ps;  // (order, start)
         std::vector<int> hs, hl;
@@ -144,9 +145,6 @@ class DevConeSet {
             hl.push_back(len);
             const cone_type_t t = b->type[k];
             if (st < 0 || len < 0 || (long long)st + len > n) fail("cone block %d lies outside the variables", k);
-            if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone)
-                fail("second-order cone block %d has %d vector slots; at most %d are supported", k, vd,
-                     kMaxThreadCone);
             for (int i = st; i < st + len; ++i) hmask[i] = t == CONE_PSD ? kMaskPsd : kMaskCone;
             if (t == CONE_PSD) {
                 psd = true;
@@ -161,19 +159,26 @@ class DevConeSet {
             bool has_fixed = false;
             for (int i = st; b->is_fixed && i < st + len; ++i) has_fixed |= b->is_fixed[i] != 0;
             if (has_fixed) {
+                if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone)
+                    fail("second-order cone block %d with fixed slots has %d vector slots; at most %d are supported", k,
+                         vd, kMaxThreadCone);
                 sec_start.push_back(st);
                 sec_vdim.push_back(vd);
                 sec_type.push_back((int)t);
                 sec_alpha.push_back(t == CONE_POWER && b->power_alpha ? b->power_alpha[k] : 0.0);
                 continue;
             }
-            const int li = t == CONE_ROTATED_SOC ? 0 : t == CONE_STANDARD_SOC ? 1 : t == CONE_EXPONENTIAL ? 2 : 3;
+            const bool long_block = (t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone;
+            const int li = t == CONE_ROTATED_SOC ? (long_block ? 4 : 0)
+                           : t == CONE_STANDARD_SOC ? (long_block ? 5 : 1)
+                           : t == CONE_EXPONENTIAL ? 2
+                                                   : 3;
             lists[li].start.push_back(st);
             lists[li].vdim.push_back(vd);
             lists[li].alpha.push_back(t == CONE_POWER && b->power_alpha ? b->power_alpha[k] : 0.0);
         }
-        ConeList *dst[4] = {&rsoc, &soc, &expc, &powc};
-        for (int li = 0; li < 4; ++li) {
+        ConeList *dst[6] = {&rsoc, &soc, &expc, &powc, &lrsoc, &lsoc};
+        for (int li = 0; li < 6; ++li) {
             const HostList &h = lists[li];
             ConeList &L = *dst[li];
             L.count = (int)h.start.size();
@@ -235,6 +240,12 @@ class DevConeSet {
         if (powc.count)
             hipLaunchKernelGGL((cone_project_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v, d,
                                powc, rw, gate);
+        if (lrsoc.count)
+            hipLaunchKernelGGL((soc_long_project_kernel<X, true>), dim3(lrsoc.count), dim3(kBlock), 0, st, v, d,
+                               (const X *)nullptr, 0.0, lrsoc, rw, gate);
+        if (lsoc.count)
+            hipLaunchKernelGGL((soc_long_project_kernel<X, false>), dim3(lsoc.count), dim3(kBlock), 0, st, v, d,
+                               (const X *)nullptr, 0.0, lsoc, rw, gate);
         if (sec.count)
             hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, (const X *)nullptr,
                                0.0, fixed, sec, gate);
@@ -258,6 +269,12 @@ class DevConeSet {
         if (powc.count)
             hipLaunchKernelGGL((cone_project_diag_q_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st, v,
                                d, q, tau, powc);
+        if (lrsoc.count)
+            hipLaunchKernelGGL((soc_long_project_kernel<X, true>), dim3(lrsoc.count), dim3(kBlock), 0, st, v, d, q,
+                               tau, lrsoc, 0, (const int *)nullptr);
+        if (lsoc.count)
+            hipLaunchKernelGGL((soc_long_project_kernel<X, false>), dim3(lsoc.count), dim3(kBlock), 0, st, v, d, q,
+                               tau, lsoc, 0, (const int *)nullptr);
         if (sec.count)
             hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, q, tau, fixed,
                                sec, (const int *)nullptr);
@@ -285,6 +302,12 @@ class DevConeSet {
         if (powc.count)
             hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_POWER>), blocks(powc.count), dim3(kBlock), 0, st,
                                dres, c, qx, aty, d, powc);
+        if (lrsoc.count)
+            hipLaunchKernelGGL((soc_long_dual_residual_kernel<X, true>), dim3(lrsoc.count), dim3(kBlock), 0, st, dres,
+                               c, qx, aty, d, lrsoc);
+        if (lsoc.count)
+            hipLaunchKernelGGL((soc_long_dual_residual_kernel<X, false>), dim3(lsoc.count), dim3(kBlock), 0, st, dres,
+                               c, qx, aty, d, lsoc);
         if (psd) {
             hipLaunchKernelGGL(psd_rc_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, scratch, dres, c, qx, aty,
                                mask, n);
@@ -297,7 +320,7 @@ class DevConeSet {
     }
 
     void release() {
-        for (ConeList *L : {&soc, &rsoc, &expc, &powc, &powall}) {
+        for (ConeList *L : {&soc, &rsoc, &expc, &powc, &lsoc, &lrsoc, &powall}) {
             void *lp[] = {L->start, L->vdim, L->alpha, L->warm, L->rwarm};
             for (void *p : lp)
                 if (p) (void)hipFree(p);

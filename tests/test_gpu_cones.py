@@ -52,9 +52,9 @@ def _orc_dual_residual(t, rc, vd, d, alpha):
     return out
 
 
-def _blocks_for(t, rng, count):
+def _blocks_for(t, rng, count, sizes=(1, 2, 3, 7, 16, 33, 64)):
     if t in (SOC, RSOC):
-        return [(t, int(v)) for v in rng.choice([1, 2, 3, 7, 16, 33, 64], size=count)]
+        return [(t, int(v)) for v in rng.choice(list(sizes), size=count)]
     if t == POW:
         return [(t, 1, float(a)) for a in rng.uniform(0.05, 0.95, size=count)]
     if t == PSD:
@@ -179,14 +179,14 @@ def test_sparse_q_with_soc_e2e():
     assert abs(g.pobj + 1.5) < 1e-3
 
 
-def random_conic(seed, types, q=None):
+def random_conic(seed, types, q=None, soc_sizes=(1, 2, 3, 7, 16, 33, 64), count=6):
     """Strictly primal and dual feasible conic program: min c'x (+ x'Qx/2),
     A x = A x_int, x in K on the cone slots, 0 <= x <= 10 on the free slots,
     with x_int interior to K and c = A'y0 + s0, s0 interior to K*."""
     rng = np.random.default_rng(seed)
     blocks = []
     for t in types:
-        blocks += _blocks_for(t, rng, 6)
+        blocks += _blocks_for(t, rng, count, soc_sizes)
     specs, n = _layout(blocks, gap=2)
     xi = rng.uniform(1.0, 2.0, n)
     s0 = rng.uniform(0.5, 1.5, n)
@@ -280,11 +280,80 @@ def test_diag_q_with_psd_uses_inner_solve():
     assert g.inner_iterations > g.iterations
 
 
+# ---- second-order blocks above 64 vector slots (one workgroup per block) ----
+
+LONG = (65, 100, 257, 1000, 4096)
+
+
+def _long_point(specs, n, rng):
+    """Random point whose long blocks are, in turn, outside the cone, inside
+    it and in its polar, so that every branch of the projection runs."""
+    x = rng.standard_normal(n) * rng.choice([0.01, 1.0, 100.0], size=n)
+    for i, sp_ in enumerate(specs):
+        t, st, vd = sp_[0], sp_[1], sp_[2]
+        nrm = np.linalg.norm(x[st:st + vd + 1])
+        if i % 3 == 1:
+            x[st + vd + 1] = 2.0 * nrm + 1.0 if t == SOC else nrm * nrm + 1.0
+            if t == RSOC:
+                x[st + vd] = x[st + vd + 1]
+        elif i % 3 == 2:
+            x[st + vd + 1] = -(2.0 * nrm + 1.0)
+            if t == RSOC:
+                x[st + vd] = x[st + vd + 1]
+    return x
+
+
+@pytest.mark.parametrize("t", [SOC, RSOC])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_long_block_projection_matches_oracle(t, uniform):
+    rng = np.random.default_rng(300 + 10 * t + uniform)
+    blocks = [(t, v) for v in rng.permutation(list(LONG) * 3 + [3, 64])]
+    specs, n = _layout(blocks)
+    d = _scaling(specs, n, rng, uniform)
+    x = _long_point(specs, n, rng)
+    got = pdhcg_amd.project_cones(specs, x, d)
+    for s in specs:
+        L = _slots(s[0], s[2])
+        sl = slice(s[1], s[1] + L)
+        exp = orc_project(s[0], x[sl], s[2], d[sl], 0.0)
+        scale = 1.0 + np.max(np.abs(x[sl]))
+        assert np.max(np.abs(got[sl] - exp)) <= 1e-10 * scale, (s[:3], np.max(np.abs(got[sl] - exp)))
+
+
+@pytest.mark.parametrize("t", [SOC, RSOC])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_long_block_dual_residual_matches_oracle(t, uniform):
+    rng = np.random.default_rng(400 + 10 * t + uniform)
+    blocks = [(t, v) for v in rng.permutation(list(LONG) * 3 + [5])]
+    specs, n = _layout(blocks)
+    d = _scaling(specs, n, rng, uniform)
+    rc = _long_point(specs, n, rng)
+    got = pdhcg_amd.cone_dual_residual(specs, rc, d)
+    for s in specs:
+        L = _slots(s[0], s[2])
+        sl = slice(s[1], s[1] + L)
+        exp = _orc_dual_residual(s[0], rc[sl].copy(), s[2], d[sl].copy(), 0.0)
+        scale = 1.0 + np.max(np.abs(rc[sl]))
+        assert np.max(np.abs(got[sl] - exp)) <= 1e-10 * scale * np.max(d[sl]), (s[:3], np.max(np.abs(got[sl] - exp)))
+
+
+def test_long_blocks_e2e():
+    p = random_conic(61, (SOC, RSOC), soc_sizes=(65, 200, 700), count=2)
+    _device_vs_oracle(p, 1e-6, iteration_limit=200000)
+
+
+def test_long_blocks_diag_q_e2e():
+    def q(n, rng):
+        return sp.diags(rng.uniform(0.1, 2.0, n)).tocsr()
+    p = random_conic(67, (SOC, RSOC), q=q, soc_sizes=(65, 300), count=2)
+    _device_vs_oracle(p, 1e-6, iteration_limit=200000)
+
+
 def test_unsupported_cone_configurations_are_rejected():
-    # second-order blocks longer than the per-thread limit (64 vector slots)
+    # fixed slots in a second-order block longer than 64 vector slots
     n = 67
     p = QPData(c=np.zeros(n), A=sp.csr_matrix(np.eye(1, n)), con_lb=[1.0], con_ub=[1.0], var_lb=[-1e30] * n,
-               var_ub=[1e30] * n, cones=[(SOC, 0, 65)])
+               var_ub=[1e30] * n, cones=[(SOC, 0, 65)], cone_fixed=[(0, 3, 0.5)])
     assert pdhcg_amd.solve(p, eps=1e-6) is None
 
 
