@@ -25,8 +25,9 @@ def test_affine_cone_known_answers(case):
     assert abs(g.pobj - o.pobj) <= 1e-6 * (1 + abs(o.pobj)), (name, g.pobj, o.pobj)
 
 
-@pytest.mark.parametrize("n,k,factor", [(200, 20, 1.5), (1000, 50, 1.5), (60, 8, 1.0)])
+@pytest.mark.parametrize("n,k,factor", [(200, 20, 1.5), (1000, 50, 1.5), (60, 8, 1.0), (400, 120, 1.5)])
 def test_portfolio_tracking_matches_oracle(n, k, factor):
+    # k = 120: the tracking-error cone has more than 64 vector slots (one workgroup per block)
     p = workloads.portfolio_tracking(n, k, seed=7 + n, te_factor=factor)
     eps = 1e-6
     g = pdhcg_amd.solve(p, eps=eps, iteration_limit=200000)
