@@ -597,6 +597,20 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
          t += (long long)gridDim.x * blockDim.x) {
         const long long i0 = kRtuCols * t;
+        // the epilogue's operands are loaded before the pass over R, so that
+        // their latency overlaps it
+        double ea[kRtuCols], eb[kRtuCols];
+#pragma unroll
+        for (int q = 0; q < kRtuCols; ++q) {
+            const long long i = i0 + q;
+            ea[q] = eb[q] = 0.0;
+            if (i < n && qdiag) {
+                ea[q] = (double)qdiag[i];
+                eb[q] = (double)x[i];
+            } else if (i < n && accumulate) {
+                ea[q] = (double)qx[i];
+            }
+        }
         double s[kRtuCols];
         rtu_cols(R, u, i0, k, s);
 #pragma unroll
@@ -604,8 +618,8 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
             const long long i = i0 + q;
             if (i >= n) break;
             double v = s[q];
-            if (qdiag) v = v + (double)qdiag[i] * (double)x[i];
-            else if (accumulate) v = v + (double)qx[i];
+            if (qdiag) v = v + ea[q] * eb[q];
+            else if (accumulate) v = v + ea[q];
             qx[i] = (X)v;
         }
     }
@@ -627,6 +641,22 @@ __global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const doub
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
          t += (long long)gridDim.x * blockDim.x) {
         const long long i0 = kRtuCols * t;
+        // the epilogue's operands are loaded before the pass over R, so that
+        // their latency overlaps it
+        double exp_[kRtuCols], eq[kRtuCols], ec[kRtuCols], ex[kRtuCols], eg[kRtuCols], ed[kRtuCols];
+#pragma unroll
+        for (int q = 0; q < kRtuCols; ++q) {
+            const long long i = i0 + q;
+            exp_[q] = eq[q] = ec[q] = ex[q] = eg[q] = ed[q] = 0.0;
+            if (i < n) {
+                exp_[q] = (double)xp[i];
+                eq[q] = qdiag ? (double)qdiag[i] : qx_sparse ? (double)qx_sparse[i] : 0.0;
+                ec[q] = (double)c[i] - (double)aty[i];
+                ex[q] = (double)x[i];
+                eg[q] = (double)grad[i];
+                ed[q] = (double)dirn[i];
+            }
+        }
         double s[kRtuCols];
         rtu_cols(R, u, i0, k, s);
 #pragma unroll
@@ -634,14 +664,14 @@ __global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const doub
             const long long i = i0 + q;
             if (i >= n) break;
             double sq = s[q];
-            const double xpi = (double)xp[i];
-            if (qdiag) sq = sq + (double)qdiag[i] * xpi;
-            else if (qx_sparse) sq = sq + (double)qx_sparse[i];
+            const double xpi = exp_[q];
+            if (qdiag) sq = sq + eq[q] * xpi;
+            else if (qx_sparse) sq = sq + eq[q];
             const double qv = (double)(X)sq;  // the stored product's rounding, as bb_gradient_kernel sees it
-            const double cur = qv + (double)c[i] - (double)aty[i] + inv_tau * (xpi - (double)x[i]);
-            const double dg = (double)(X)(cur - (double)grad[i]);
+            const double cur = qv + ec[q] + inv_tau * (xpi - ex[q]);
+            const double dg = (double)(X)(cur - eg[q]);
             grad[i] = (X)cur;
-            v[0] += (double)dirn[i] * dg;
+            v[0] += ed[q] * dg;
         }
     }
     grid_partials<1, 0>(g, v);
