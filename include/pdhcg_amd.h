@@ -97,8 +97,9 @@ int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const 
  * with pdhcg_result_free), or NULL when problem i has something the batch does
  * not handle (message on stderr): variable cone blocks, a sparse Q with
  * off-diagonal entries or of type PDHCG_SPARSE_Q, quadratic constraints,
- * affine PSD cones, affine second-order cones above 64 vector slots, a
- * low-rank term above rank 64.  params (NULL: defaults) apply to every
+ * affine PSD cones, a low-rank term above rank 256.  Affine second-order
+ * cones of any length are taken (above 64 vector slots the problem's
+ * workgroup projects them one at a time).  params (NULL: defaults) apply to every
  * problem; the time limit is checked between windows of iterations.  Returns
  * 0, or -1 for invalid arguments or a device error. */
 int pdhcg_solve_batch(const qp_problem_t *const *problems, int count, const pdhg_parameters_t *params,

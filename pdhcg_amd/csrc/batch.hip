@@ -88,12 +88,10 @@ const char *unsupported(const qp_problem_t *p) {
     if (!p) return "problem is NULL";
     if (p->cones.num_cones > 0) return "variable cone blocks (use solve_qp_problem)";
     if (p->num_quadratic_constraints > 0) return "quadratic constraints";
-    if (p->num_rank_lowrank_obj > kMaxRank) return "a low-rank term of rank above 64";
+    if (p->num_rank_lowrank_obj > kMaxRank) return "a low-rank term of rank above 256";
     for (int c = 0; c < p->affine_cones.num_cones; ++c) {
         const cone_type_t ct = p->affine_cones.type[c];
         if (ct == CONE_PSD) return "an affine PSD cone";
-        if ((ct == CONE_STANDARD_SOC || ct == CONE_ROTATED_SOC) && p->affine_cones.v_dim[c] > kMaxThreadCone)
-            return "an affine second-order cone of more than 64 vector slots";
     }
     return nullptr;
 }
@@ -186,9 +184,19 @@ bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const
     const cone_blocks_t *ac = sp.affine_cones;
     I.ncones = (ac && m > 0) ? ac->num_cones : 0;
     I.has_affine = I.ncones > 0;
-    for (int c = 0; c < I.ncones; ++c)
-        k.cones.push_back(Cone{(int)ac->type[c], ac->start_idx[c], ac->v_dim[c],
-                               ac->power_alpha ? ac->power_alpha[c] : 0.0});
+    // cones in input order, long second-order cones moved to the end (the
+    // workgroup projects those one at a time); the per-cone values follow
+    // this order
+    I.nlong = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < I.ncones; ++c) {
+            const bool lng = (ac->type[c] == CONE_STANDARD_SOC || ac->type[c] == CONE_ROTATED_SOC) &&
+                             ac->v_dim[c] > kMaxThreadCone;
+            if (lng != (pass == 1)) continue;
+            I.nlong += lng ? 1 : 0;
+            k.cones.push_back(Cone{(int)ac->type[c], ac->start_idx[c], ac->v_dim[c],
+                                   ac->power_alpha ? ac->power_alpha[c] : 0.0});
+        }
     // step size and primal weight (reference src/pdhg_core_op.cu:933-971)
     if (sp.A.nnz() == 0) I.step = 1.0;
     else {

@@ -30,7 +30,7 @@ namespace batch {
 
 constexpr int kBT = 512;  // threads per problem (two waves per SIMD at 248 VGPRs)
 constexpr int kBW = kBT / kWave;
-constexpr int kMaxRank = 64;
+constexpr int kMaxRank = 256;  // low-rank terms up to this rank (R x, u live in the slab)
 
 enum QKind { kQNone = 0, kQDiag = 1, kQLowRank = 2 };
 
@@ -45,6 +45,7 @@ struct Params {
 // One problem: sizes, pool offsets and scaled-problem scalars.
 struct Inst {
     int n, m, k, qkind, dkind, has_affine, ncones, cone0;
+    int nlong;                     // SOC/RSOC cones above kMaxThreadCone slots: the last nlong cones
     int a_rp, a_ci, at_rp, at_ci;  // int-pool offsets of [A;F] and its transpose
     int a_long, at_long;           // 1: a wavefront per row, 0: a thread per row
     long long a_v, at_v;           // value-pool offsets of the matrix values
@@ -218,7 +219,8 @@ struct Ctx {
     // (reference src/cone_dispatch.cu:38-57 on the row space).
     __device__ __noinline__ void project_affine(double *v, double *warm) const {
         const double *d = mv(wAFD);
-        for (int c = threadIdx.x; c < I.ncones; c += kBT) {
+        const int nshort = I.ncones - I.nlong;
+        for (int c = threadIdx.x; c < nshort; c += kBT) {
             const Cone cn = cones[c];
             const int s = cn.start;
             if (cn.type == CONE_STANDARD_SOC || cn.type == CONE_ROTATED_SOC) {
@@ -242,6 +244,19 @@ struct Ctx {
             }
         }
         __syncthreads();
+        // long second-order cones: the whole workgroup on one cone at a time
+        for (int c = nshort; c < I.ncones; ++c) {
+            const Cone cn = cones[c];
+            double *pv = v + cn.start;
+            const double *pd = d + cn.start;
+            auto P = [&](int m) { return pv[m]; };
+            auto E = [&](int m) { return pd[m]; };
+            auto Set = [&](int m, double val) { pv[m] = val; };
+            auto sum = [&](double x) { return red.sum(x); };
+            if (cn.type == CONE_ROTATED_SOC) soc_project_coop<true>(P, E, Set, cn.vdim, warm + c, sum, false);
+            else soc_project_coop<false>(P, E, Set, cn.vdim, warm + c, sum, false);
+            __syncthreads();
+        }
     }
 };
 

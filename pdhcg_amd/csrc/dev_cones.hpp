@@ -467,20 +467,21 @@ template <int NV> struct WgSum {
 };
 
 // soc_multiplier with f summed over the workgroup; p(m), h(m) give slot m's
-// value and relative scaling.  Called by every thread of the workgroup.
-template <class Pf, class Hf>
-__device__ double soc_multiplier_wg(Pf p, Hf h, int k, double w, double dw, double z, double *warm, WgSum<1> &red) {
+// value and relative scaling, sum(x) the workgroup total of x (every thread
+// receives it).  Called by every thread of the workgroup.
+template <class Pf, class Hf, class Sum>
+__device__ double soc_multiplier_wg(Pf p, Hf h, int k, double w, double dw, double z, double *warm, Sum sum) {
     auto f = [&](double lam) {
-        double v[1] = {0.0};
+        double acc = 0.0;
         for (int m = threadIdx.x; m < k; m += blockDim.x) {
             const double hm = h(m);
             const double t = p(m) * hm / (hm * hm + 2.0 * lam);
-            v[0] += t * t;
+            acc += t * t;
         }
-        red(v);
+        acc = sum(acc);
         const double tw = w * dw / (dw * dw + 2.0 * lam);
         const double zt = z / (1.0 - 2.0 * lam);
-        return v[0] + tw * tw - zt * zt;
+        return acc + tw * tw - zt * zt;
     };
     double lo, hi;
     const bool zpos = z > 0.0;
@@ -516,25 +517,13 @@ __device__ double soc_multiplier_wg(Pf p, Hf h, int k, double w, double dw, doub
     return lam;
 }
 
-// Projection of long blocks onto D K (q == null) or, for the diagonal-Q prox,
-// onto D K in the metric diag(w), w = 1 + tau q, as W^-1/2 P_{W^1/2 D K}(W^1/2 v)
-// (cone_project_kernel / cone_project_diag_q_kernel).  Grid: L.count workgroups.
-template <class X, bool Rotated>
-__global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const double *d, const X *q, double tau,
-                                                                   ConeList L, int use_rwarm, const int *gate) {
-    if (gate && *gate == 0) return;
-    __shared__ WgSum<2> red2;
-    __shared__ WgSum<1> red1;
-    const int b = blockIdx.x;
-    const int s = L.start[b], k = L.vdim[b];
-    double *warm = use_rwarm ? L.rwarm + b : L.warm + b;
-    auto sw = [&](int m) {
-        if (!q) return 1.0;
-        const double w = 1.0 + tau * (double)q[s + m];
-        return sqrt(w > 0.0 ? w : 1.0);
-    };
-    auto P = [&](int m) { return (double)v[s + m] * sw(m); };
-    auto E = [&](int m) { return d[s + m] * sw(m); };
+// Projection of one long SOC/RSOC block onto E K (E = diag(e)) by the whole
+// workgroup, the algorithm of soc_project_block: P(m) reads slot m (m < k + 2),
+// E(m) its scaling, Set(m, v) writes it; sum(x) is a workgroup sum.  With
+// write_all, a point already in the cone is written back as well.  Every
+// slot is read before any is written (each reduction ends with a barrier).
+template <bool Rotated, class Pf, class Ef, class Setf, class Sum>
+__device__ void soc_project_coop(Pf P, Ef E, Setf Set, int k, double *warm, Sum sum, bool write_all) {
     const double a = P(k), bb = P(k + 1);
     const double w = Rotated ? (a - bb) * kInvSqrt2 : a;
     const double z = Rotated ? (a + bb) * kInvSqrt2 : bb;
@@ -542,23 +531,21 @@ __global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const do
     const double dw = Rotated ? 1.0 : E(k) / dref;
     bool uniform = Rotated || dw == 1.0;
     {
-        double nu[2] = {0.0, 0.0};
+        double nu = 0.0;
         for (int m = threadIdx.x; m < k; m += blockDim.x)
-            if (E(m) != dref) nu[0] = 1.0;
-        red2(nu);
-        uniform = uniform && nu[0] == 0.0;
+            if (E(m) != dref) nu = 1.0;
+        uniform = uniform && sum(nu) == 0.0;
     }
     // final transform: mode 0 keep, 1 zero, 2 uniform scale sc, 3 weighted with lam
     int mode;
     double sc = 0.0, lam = 0.0, wn = 0.0, zn = 0.0;
     if (uniform) {
-        double ss[2] = {0.0, 0.0};
+        double ss = 0.0;
         for (int m = threadIdx.x; m < k; m += blockDim.x) {
             const double pm = P(m);
-            ss[0] += pm * pm;
+            ss += pm * pm;
         }
-        red2(ss);
-        const double r = sqrt(w * w + ss[0]);
+        const double r = sqrt(w * w + sum(ss));
         if (r <= z) mode = 0;
         else if (r <= -z) mode = 1;
         else {
@@ -569,25 +556,25 @@ __global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const do
         }
     } else {
         auto h = [&](int m) { return E(m) / dref; };
-        double rr[2] = {0.0, 0.0};  // sum (p/h)^2, sum (p h)^2 over the vector slots
+        double r0 = 0.0, r1 = 0.0;  // sum (p/h)^2, sum (p h)^2 over the vector slots
         for (int m = threadIdx.x; m < k; m += blockDim.x) {
             const double hm = h(m), pm = P(m);
-            rr[0] += (pm / hm) * (pm / hm);
-            rr[1] += (pm * hm) * (pm * hm);
+            r0 += (pm / hm) * (pm / hm);
+            r1 += (pm * hm) * (pm * hm);
         }
-        red2(rr);
-        const double ri = (w / dw) * (w / dw) + rr[0], rp = (w * dw) * (w * dw) + rr[1];
+        r0 = sum(r0);
+        r1 = sum(r1);
+        const double ri = (w / dw) * (w / dw) + r0, rp = (w * dw) * (w * dw) + r1;
         if (sqrt(ri) <= z) mode = 0;
         else if (sqrt(rp) <= -z) mode = 1;
         else {
             mode = 3;
-            lam = soc_multiplier_wg(P, h, k, w, dw, z, warm, red1);
+            lam = soc_multiplier_wg(P, h, k, w, dw, z, warm, sum);
             wn = Rotated ? w / (1.0 + 2.0 * lam) : w * dw * dw / (dw * dw + 2.0 * lam);
             zn = z / (1.0 - 2.0 * lam);
         }
     }
-    if (mode == 0 && !q) return;
-    // every slot is read above before any is written: the reductions end with a barrier
+    if (mode == 0 && !write_all) return;
     for (int m = threadIdx.x; m < k + 2; m += blockDim.x) {
         double pm = P(m);
         if (mode == 1) pm = 0.0;
@@ -601,8 +588,35 @@ __global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const do
             if (Rotated) pm = m == k ? (zn + wn) * kInvSqrt2 : (zn - wn) * kInvSqrt2;
             else pm = m == k ? wn : zn;
         }
-        v[s + m] = (X)(pm / sw(m));
+        Set(m, pm);
     }
+}
+
+// Projection of long blocks onto D K (q == null) or, for the diagonal-Q prox,
+// onto D K in the metric diag(w), w = 1 + tau q, as W^-1/2 P_{W^1/2 D K}(W^1/2 v)
+// (cone_project_kernel / cone_project_diag_q_kernel).  Grid: L.count workgroups.
+template <class X, bool Rotated>
+__global__ void __launch_bounds__(kBlock) soc_long_project_kernel(X *v, const double *d, const X *q, double tau,
+                                                                   ConeList L, int use_rwarm, const int *gate) {
+    if (gate && *gate == 0) return;
+    __shared__ WgSum<1> red;
+    const int b = blockIdx.x;
+    const int s = L.start[b], k = L.vdim[b];
+    double *warm = use_rwarm ? L.rwarm + b : L.warm + b;
+    auto sw = [&](int m) {
+        if (!q) return 1.0;
+        const double w = 1.0 + tau * (double)q[s + m];
+        return sqrt(w > 0.0 ? w : 1.0);
+    };
+    auto P = [&](int m) { return (double)v[s + m] * sw(m); };
+    auto E = [&](int m) { return d[s + m] * sw(m); };
+    auto Set = [&](int m, double val) { v[s + m] = (X)(val / sw(m)); };
+    auto sum = [&](double x) {
+        double t[1] = {x};
+        red(t);
+        return t[0];
+    };
+    soc_project_coop<Rotated>(P, E, Set, k, warm, sum, q != nullptr);
 }
 
 // Dual residual of long blocks (cone_dual_residual_kernel): (rc - P_{K*}(rc)) d
@@ -614,6 +628,11 @@ __global__ void __launch_bounds__(kBlock) soc_long_dual_residual_kernel(double *
     __shared__ WgSum<1> red1;
     const int b = blockIdx.x;
     const int s = L.start[b], k = L.vdim[b];
+    auto sum1 = [&](double x) {
+        double t[1] = {x};
+        red1(t);
+        return t[0];
+    };
     auto rc = [&](int m) {
         const int i = s + m;
         return (double)c[i] + (qx ? (double)qx[i] : 0.0) - (double)aty[i];
@@ -687,7 +706,7 @@ __global__ void __launch_bounds__(kBlock) soc_long_dual_residual_kernel(double *
             for (int m = threadIdx.x; m < k + 2; m += blockDim.x) dres[s + m] = rc(m) * D(m);
             return;
         }
-        const double lam = soc_multiplier_wg(rc, h, k, rw, ew, rz, L.rwarm + b, red1);
+        const double lam = soc_multiplier_wg(rc, h, k, rw, ew, rz, L.rwarm + b, sum1);
         for (int m = threadIdx.x; m < k; m += blockDim.x) {
             const double hm = h(m), pm = rc(m);
             dres[s + m] = (pm - pm * hm * hm / (hm * hm + 2.0 * lam)) * D(m);

@@ -83,3 +83,16 @@ def test_batch_matches_single_device_solve():
     s = pdhcg_amd.solve(p, eps=EPS, iteration_limit=200000)
     assert b.status == s.status == "OPTIMAL"
     assert abs(b.pobj - s.pobj) <= 5 * EPS * (1 + abs(s.pobj))
+
+
+def test_batch_portfolios_with_long_cones_match_oracle():
+    # 65..120 factors: tracking-error cones above 64 vector slots, projected
+    # by the whole workgroup; mixed with short-cone portfolios in one batch
+    problems = workloads.portfolio_batch(3, seed=31, n_range=(100, 300), k_range=(65, 120))
+    problems += workloads.portfolio_batch(2, seed=32, n_range=(100, 300), k_range=(10, 40))
+    sols = pdhcg_amd.solve_batch(problems, eps=EPS, iteration_limit=200000, time_limit=60.0)
+    for p, g in zip(problems, sols):
+        o = oracle_solve(p, eps=EPS, iteration_limit=200000)
+        _agree(p, g, o, affine=True)
+        te = np.linalg.norm(np.sqrt(p.D) * (p.R @ g.x))
+        assert te <= p.g[-1] * (1 + 1e-4)

@@ -337,9 +337,14 @@ def test_long_block_dual_residual_matches_oracle(t, uniform):
         assert np.max(np.abs(got[sl] - exp)) <= 1e-10 * scale * np.max(d[sl]), (s[:3], np.max(np.abs(got[sl] - exp)))
 
 
-def test_long_blocks_e2e():
+@pytest.mark.parametrize("precision", [pdhcg_amd.PRECISION_FP64, pdhcg_amd.PRECISION_MIXED])
+def test_long_blocks_e2e(precision):
     p = random_conic(61, (SOC, RSOC), soc_sizes=(65, 200, 700), count=2)
-    _device_vs_oracle(p, 1e-6, iteration_limit=200000)
+    eps = 1e-6
+    g = pdhcg_amd.solve(p, precision, eps=eps, iteration_limit=200000)
+    o = oracle_solve(p, eps=eps, iteration_limit=200000)
+    assert g.status == o.status == "OPTIMAL", (g.status, o.status)
+    assert abs(g.pobj - o.pobj) <= 5 * eps * (1 + abs(o.pobj)), (g.pobj, o.pobj)
 
 
 def test_long_blocks_diag_q_e2e():
