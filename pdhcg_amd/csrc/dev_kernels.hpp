@@ -553,39 +553,45 @@ __global__ void __launch_bounds__(kBlock) rx_fold_kernel(const double *partials,
     if (threadIdx.x == 0) *counter = 0u;
 }
 
-// Two adjacent columns i0, i0 + 1 of R'u (one 16-byte load per row of R in
-// FP64), each column's sum over the rows in order j = 0 .. k - 1.  Columns
-// past n lie in the zero padding of the last tile.  On MI355X this pattern
-// streams the tiles at about 6.4 TB/s against 6.2 TB/s for one column per
-// lane; four columns per lane need too many registers for ten rows in flight
-// (4.0 TB/s).
-constexpr int kRtuCols = 2;
+// Adjacent columns i0 .. i0 + RtuCols<V> - 1 of R'u, one 16-byte load per row
+// of R (two FP64 or four FP32 columns), each column's sum over the rows in
+// order j = 0 .. k - 1.  Columns past n lie in the zero padding of the last
+// tile.  On MI355X this pattern streams the tiles at about 6.4 TB/s in FP64,
+// against 6.2 TB/s for one column per lane; four FP64 columns per lane (32
+// bytes) need too many registers for ten rows in flight (4.0 TB/s).  The
+// loaded values stay in the storage type until the multiply-add.
+template <class V> struct RtuCols {
+    static constexpr int value = 16 / (int)sizeof(V);
+};
 template <class V>
-__device__ __forceinline__ void rtu_cols(const V *R, const double *u, long long i0, int k, double (&s)[kRtuCols]) {
+__device__ __forceinline__ void rtu_cols(const V *R, const double *u, long long i0, int k,
+                                         double (&s)[RtuCols<V>::value]) {
+    constexpr int RC = RtuCols<V>::value;
+    typedef V T __attribute__((ext_vector_type(RC)));
     constexpr int JU = 10;  // rows of R loaded before the multiply-adds
     const V *col = R + rx_tile_index(0, i0, k);  // row j of these columns at col[j * kRxWaveCols]
 #pragma unroll
-    for (int q = 0; q < kRtuCols; ++q) s[q] = 0.0;
+    for (int q = 0; q < RC; ++q) s[q] = 0.0;
     int j = 0;
     for (; j + JU <= k; j += JU) {
-        double r[JU][kRtuCols];
+        T r[JU];
 #pragma unroll
-        for (int a = 0; a < JU; ++a) stream_load2(col + (size_t)(j + a) * kRxWaveCols, r[a]);
+        for (int a = 0; a < JU; ++a)
+            r[a] = __builtin_nontemporal_load(reinterpret_cast<const T *>(col + (size_t)(j + a) * kRxWaveCols));
 #pragma unroll
         for (int a = 0; a < JU; ++a)
 #pragma unroll
-            for (int q = 0; q < kRtuCols; ++q) s[q] += r[a][q] * u[j + a];
+            for (int q = 0; q < RC; ++q) s[q] += (double)r[a][q] * u[j + a];
     }
     for (; j < k; ++j) {
-        double r[kRtuCols];
-        stream_load2(col + (size_t)j * kRxWaveCols, r);
+        const T r = __builtin_nontemporal_load(reinterpret_cast<const T *>(col + (size_t)j * kRxWaveCols));
 #pragma unroll
-        for (int q = 0; q < kRtuCols; ++q) s[q] += r[q] * u[j];
+        for (int q = 0; q < RC; ++q) s[q] += (double)r[q] * u[j];
     }
 }
 
-// qx = R'u (+ qx, or + qdiag .* x)  with R dense in column tiles, kRtuCols
-// adjacent columns per thread (grid over ceil(n / kRtuCols) threads).
+// qx = R'u (+ qx, or + qdiag .* x)  with R dense in column tiles,
+// RtuCols<V> adjacent columns per thread (grid over ceil(n / RtuCols<V>) threads).
 // Gated launches (inside the BB loop) are a separate instantiation so that
 // profiles separate them from the always-running ones.
 template <class V, class X, bool Gated>
@@ -593,15 +599,16 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
                                                             const X *qdiag, const X *x, int accumulate,
                                                             const int *gate) {
     if (Gated && *gate == 0) return;
-    const long long groups = ((long long)n + kRtuCols - 1) / kRtuCols;
+    constexpr int RC = RtuCols<V>::value;
+    const long long groups = ((long long)n + RC - 1) / RC;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
          t += (long long)gridDim.x * blockDim.x) {
-        const long long i0 = kRtuCols * t;
+        const long long i0 = RC * t;
         // the epilogue's operands are loaded before the pass over R, so that
         // their latency overlaps it
-        double ea[kRtuCols], eb[kRtuCols];
+        double ea[RC], eb[RC];
 #pragma unroll
-        for (int q = 0; q < kRtuCols; ++q) {
+        for (int q = 0; q < RC; ++q) {
             const long long i = i0 + q;
             ea[q] = eb[q] = 0.0;
             if (i < n && qdiag) {
@@ -611,10 +618,10 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
                 ea[q] = (double)qx[i];
             }
         }
-        double s[kRtuCols];
+        double s[RC];
         rtu_cols(R, u, i0, k, s);
 #pragma unroll
-        for (int q = 0; q < kRtuCols; ++q) {
+        for (int q = 0; q < RC; ++q) {
             const long long i = i0 + q;
             if (i >= n) break;
             double v = s[q];
@@ -628,24 +635,25 @@ __global__ void __launch_bounds__(kBlock) rtu_dense_kernel(const V *R, const dou
 // BB body on the dense low-rank path: the R'u pass fused with the gradient
 // step of bb_gradient_kernel.  Qx_p = R'u (+ diag(Q) x_p, or + the sparse-Q
 // product already in qx_sparse); g = Qx_p + c - A'y + (x_p - x)/tau;
-// partial of s'(g - g_old); grad = g.  Qx_p itself is not stored.  kRtuCols
-// adjacent columns per thread, as rtu_dense_kernel.
+// partial of s'(g - g_old); grad = g.  Qx_p itself is not stored.
+// RtuCols<V> adjacent columns per thread, as rtu_dense_kernel.
 template <class V, class X>
 __global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const double *u, int n, int k,
                                                            const X *qx_sparse, const X *qdiag, const X *xp, const X *x,
                                                            const X *c, const X *aty, X *grad, const X *dirn,
                                                            double inv_tau, GridRed g, const DevScalars *ds) {
     if (ds->bb_active == 0) return;
+    constexpr int RC = RtuCols<V>::value;
     double v[1] = {0.0};
-    const long long groups = ((long long)n + kRtuCols - 1) / kRtuCols;
+    const long long groups = ((long long)n + RC - 1) / RC;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < groups;
          t += (long long)gridDim.x * blockDim.x) {
-        const long long i0 = kRtuCols * t;
+        const long long i0 = RC * t;
         // the epilogue's operands are loaded before the pass over R, so that
         // their latency overlaps it
-        double exp_[kRtuCols], eq[kRtuCols], ec[kRtuCols], ex[kRtuCols], eg[kRtuCols], ed[kRtuCols];
+        double exp_[RC], eq[RC], ec[RC], ex[RC], eg[RC], ed[RC];
 #pragma unroll
-        for (int q = 0; q < kRtuCols; ++q) {
+        for (int q = 0; q < RC; ++q) {
             const long long i = i0 + q;
             exp_[q] = eq[q] = ec[q] = ex[q] = eg[q] = ed[q] = 0.0;
             if (i < n) {
@@ -657,10 +665,10 @@ __global__ void __launch_bounds__(kBlock) rtu_grad_kernel(const V *R, const doub
                 ed[q] = (double)dirn[i];
             }
         }
-        double s[kRtuCols];
+        double s[RC];
         rtu_cols(R, u, i0, k, s);
 #pragma unroll
-        for (int q = 0; q < kRtuCols; ++q) {
+        for (int q = 0; q < RC; ++q) {
             const long long i = i0 + q;
             if (i >= n) break;
             double sq = s[q];

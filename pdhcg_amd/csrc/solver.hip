@@ -979,7 +979,7 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
                                    k_, rx_partials_, rx_chunks_, gate);
                 hipLaunchKernelGGL(rx_fold_kernel<true>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
                                    rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
-                hipLaunchKernelGGL((rtu_dense_kernel<V, X, true>), dim3(full_grid(((long long)n_ + kRtuCols - 1) / kRtuCols)), dim3(kBlock), 0, st_, Rd_, u_,
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, true>), dim3(full_grid(((long long)n_ + RtuCols<V>::value - 1) / RtuCols<V>::value)), dim3(kBlock), 0, st_, Rd_, u_,
                                    n_, k_, qx, qd, x, acc, gate);
             } else {
                 const int t0 = timer_.start(kTimedRx, st_);
@@ -989,7 +989,7 @@ template <class V, class X> void Solver<V, X>::q_product(const X *x, X *qx, cons
                 hipLaunchKernelGGL(rx_fold_kernel<false>, dim3(k_), dim3(kBlock), 0, st_, rx_partials_, rx_chunks_, k_,
                                    rx_sums_, rx_counter_, u_, ddiag_, ddense_, dkind_, gate);
                 const int t1 = timer_.start(kTimedRtu, st_);
-                hipLaunchKernelGGL((rtu_dense_kernel<V, X, false>), dim3(full_grid(((long long)n_ + kRtuCols - 1) / kRtuCols)), dim3(kBlock), 0, st_, Rd_,
+                hipLaunchKernelGGL((rtu_dense_kernel<V, X, false>), dim3(full_grid(((long long)n_ + RtuCols<V>::value - 1) / RtuCols<V>::value)), dim3(kBlock), 0, st_, Rd_,
                                    u_, n_, k_, qx, qd, x, acc, gate);
                 timer_.stop(kTimedRtu, t1, st_);
             }
@@ -1039,7 +1039,7 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     const bool plus = qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q;
     const bool fused_body = r_dense_ && (qtype_ == PDHCG_LOW_RANK_Q || plus);
     const int max_bodies = inner_limit_ - 1;
-    const int nb_grad = fused_body ? grid(((long long)n_ + kRtuCols - 1) / kRtuCols) : nb;  // rtu_grad: kRtuCols columns per thread
+    const int nb_grad = fused_body ? grid(((long long)n_ + RtuCols<V>::value - 1) / RtuCols<V>::value) : nb;  // rtu_grad: RtuCols<V> columns per thread
     auto enqueue_body = [&]() {
         if (fused_body) {
             // Q x_p with the gradient step folded into the R'u pass
