@@ -281,6 +281,9 @@ def run_warm(train_count, test_count, eps):
             single_warm = (time.perf_counter() - t) / len(sample)
             rec["single_device_cold_s"] = single_cold
             rec["single_device_warm_s"] = single_warm
+            # the plain form (one cold FP64 solve per problem) against the
+            # batched warm-started form, per problem, same tolerance
+            rec["speedup_batched_warm_vs_single_cold"] = single_cold / (warm_s / test_count)
         out["tol_%g" % tol] = rec
         log(f"[bench] warm: tol {tol:g}: iterations cold {ic.mean():.0f} warm {iw.mean():.0f}")
     ws.close()
@@ -293,7 +296,9 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--compare", action="store_true", help="also run the other precision and report both")
+    ap.add_argument("--compare", dest="compare", action="store_true", default=True,
+                    help="also run the other precision and report both (default)")
+    ap.add_argument("--no-compare", dest="compare", action="store_false")
     ap.add_argument("--scale", type=float, default=1.0, help="problem size factor (1.0 = full workload)")
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--max-solve-seconds", type=float, default=60.0)
