@@ -136,26 +136,51 @@ def measured_traffic(kernel_name, workload_key):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def host_cpu():
+    """(cores this process may run on, CPU model name)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
 def run_cpu_baseline(p, eps, budget_s):
-    """FP64 CPU oracle (single thread) on a bounded sample: setup once, then
-    as many main-loop iterations as fit in budget_s."""
+    """FP64 CPU oracle on a bounded sample: setup once, then main-loop
+    iterations for budget_s / 2 on one thread and budget_s / 2 on every
+    host core (its sparse and low-rank products are row-parallel, with
+    results independent of the thread count)."""
     import ctypes as C
     from tests.oracle_lib import oracle, oracle_build, oracle_params
     L = oracle()
     h = oracle_build(p)
     prm = oracle_params(eps=eps)
+    cores, model = host_cpu()
     t = time.time()
     s = L.orc_solver_create(h.ptr, C.byref(prm))
     setup = time.time() - t
-    iters = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        L.orc_solver_run(s, 1)
-        iters += 1
-    dt = time.perf_counter() - t0
+    out = {}
+    for label, threads in (("single", 1), ("all", cores)):
+        L.orc_set_threads(threads)
+        iters = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / 2:
+            L.orc_solver_run(s, 1)
+            iters += 1
+        out[label] = dict(iters=iters, seconds=time.perf_counter() - t0, threads=L.orc_get_threads())
+    L.orc_set_threads(0)
     L.orc_solver_free(s)
     h.close()
-    return dict(iters=iters, seconds=dt, setup_s=setup)
+    return dict(setup_s=setup, cores=cores, model=model, **out)
 
 
 def run_batch(count, eps, cpu_seconds, with_cpu):
@@ -370,9 +395,13 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU oracle baseline ...")
         c = run_cpu_baseline(p, args.eps, args.cpu_seconds)
-        cpu = {"value": c["iters"] / c["seconds"], "unit": "iter/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle, {c['iters']} main-loop iterations of the same workload in {c['seconds']:.1f}s "
-                         f"(after {c['setup_s']:.1f}s setup), single thread"}
+        a, one = c["all"], c["single"]
+        cpu = {"value": a["iters"] / a["seconds"], "unit": "iter/s", "cores": a["threads"], "kind": "port",
+               "sample": f"FP64 oracle, {a['iters']} main-loop iterations of the same workload in {a['seconds']:.1f}s "
+                         f"on {a['threads']} threads (row-parallel products; after {c['setup_s']:.1f}s setup), "
+                         f"then {one['iters']} iterations in {one['seconds']:.1f}s on one thread",
+               "single_thread_value": one["iters"] / one["seconds"], "host_cores": c["cores"],
+               "cpu_model": c["model"]}
 
     out = {
         "metric": METRIC,

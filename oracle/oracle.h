@@ -32,6 +32,11 @@
 extern "C" {
 #endif
 
+/* Threads of the row-parallel products (OpenMP; results do not depend on
+ * the count); 0 restores the default. */
+void orc_set_threads(int threads);
+int orc_get_threads(void);
+
 /* ---- problem assembly (reference src/pdhcg.c, src/cone_utils.c, src/utils.cu:580-804) ---- */
 qp_problem_t *orc_create_qp_problem(const double *objective_c,
                                     const matrix_desc_t *Q_desc,

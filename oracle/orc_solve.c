@@ -6,6 +6,9 @@
  * src/pdhg_core_op.cu, src/kernels/pdhcg_kernels.cu and src/utils.cu:219-322.
  * Reductions are plain sequential sums.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "orc_cones.h"
 #include "orc_internal.h"
 
@@ -139,11 +142,33 @@ static void cone_reflection(orc_solver_t *s) {
     }
 }
 
+/* Threads used by the row-parallel products (OpenMP); 0 restores the
+   default. */
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+    static int deflt = -1;
+    if (deflt < 0) deflt = omp_get_max_threads();
+    omp_set_num_threads(threads > 0 ? threads : deflt);
+#else
+    (void)threads;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static void *xm(size_t n) { void *p = malloc(n ? n : 1); if (!p) abort(); return p; }
 static double *zeros(int n) { double *p = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); if (!p) abort(); return p; }
 
-/* y = M x over CSR */
+/* y = M x over CSR.  Rows are independent: with OpenMP they are shared
+   out among threads, each row summed in the same order as serially, so the
+   result does not depend on the thread count. */
 static void spmv(const orc_csr *M, const double *x, double *y) {
+#pragma omp parallel for schedule(dynamic, 64) if (M->nnz > 200000)
     for (int r = 0; r < M->rows; ++r) {
         double s = 0.0;
         for (int e = M->rp[r]; e < M->rp[r + 1]; ++e) s += M->v[e] * x[M->ci[e]];
@@ -230,6 +255,7 @@ static void obj_product(orc_solver_t *s, const double *x) {
         spmv(&w->Q, x, s->qx);
         spmv(&w->R, x, s->rx);
         apply_middle(s);
+#pragma omp parallel for schedule(dynamic, 64) if (s->Rt.nnz > 200000)
         for (int r = 0; r < n; ++r) {
             double acc = 0.0;
             for (int e = s->Rt.rp[r]; e < s->Rt.rp[r + 1]; ++e) acc += s->Rt.v[e] * s->rx[s->Rt.ci[e]];

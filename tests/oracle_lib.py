@@ -32,6 +32,9 @@ def oracle() -> C.CDLL:
     L.orc_solver_inner_count.argtypes = [C.c_void_p]
     L.orc_solver_free.restype = None
     L.orc_solver_free.argtypes = [C.c_void_p]
+    L.orc_set_threads.restype = None
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
     L.orc_solver_scalings.restype = None
     dp = P(C.c_double)
     L.orc_solver_scalings.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp]

@@ -182,3 +182,19 @@ def test_transpose_matches_scipy():
     T = A.T.tocsr()
     T.sort_indices()
     assert np.array_equal(trp, T.indptr) and np.array_equal(tci, T.indices) and np.array_equal(tv, T.data)
+
+
+def test_oracle_results_do_not_depend_on_thread_count():
+    # the row-parallel products keep each row's summation order
+    L = oracle()
+    p = workloads.config1(scale=0.02)
+    sols = []
+    try:
+        for threads in (1, 4):
+            L.orc_set_threads(threads)
+            sols.append(oracle_solve(p, eps=1e-4, iteration_limit=60))
+    finally:
+        L.orc_set_threads(0)
+    a, b = sols
+    assert a.iterations == b.iterations
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y) and a.pobj == b.pobj
