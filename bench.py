@@ -156,9 +156,10 @@ def host_cpu():
 
 def run_cpu_baseline(p, eps, budget_s):
     """FP64 CPU oracle on a bounded sample: setup once, then main-loop
-    iterations for budget_s / 2 on one thread and budget_s / 2 on every
-    host core (its sparse and low-rank products are row-parallel, with
-    results independent of the thread count)."""
+    iterations for budget_s / 2 on one thread and budget_s / 2 on OpenMP's
+    default thread count (OMP_NUM_THREADS, else every core; its sparse and
+    low-rank products are row-parallel, with results independent of the
+    thread count)."""
     import ctypes as C
     from tests.oracle_lib import oracle, oracle_build, oracle_params
     L = oracle()
@@ -169,7 +170,7 @@ def run_cpu_baseline(p, eps, budget_s):
     s = L.orc_solver_create(h.ptr, C.byref(prm))
     setup = time.time() - t
     out = {}
-    for label, threads in (("single", 1), ("all", cores)):
+    for label, threads in (("single", 1), ("all", 0)):
         L.orc_set_threads(threads)
         iters = 0
         t0 = time.perf_counter()
@@ -397,9 +398,10 @@ def main():
         c = run_cpu_baseline(p, args.eps, args.cpu_seconds)
         a, one = c["all"], c["single"]
         cpu = {"value": a["iters"] / a["seconds"], "unit": "iter/s", "cores": a["threads"], "kind": "port",
-               "sample": f"FP64 oracle, {a['iters']} main-loop iterations of the same workload in {a['seconds']:.1f}s "
-                         f"on {a['threads']} threads (row-parallel products; after {c['setup_s']:.1f}s setup), "
-                         f"then {one['iters']} iterations in {one['seconds']:.1f}s on one thread",
+               "sample": f"FP64 oracle after {c['setup_s']:.1f}s setup: {one['iters']} main-loop iterations of the "
+                         f"same workload in {one['seconds']:.1f}s on one thread, then {a['iters']} in "
+                         f"{a['seconds']:.1f}s on {a['threads']} threads (row-parallel products and elementwise "
+                         f"loops, serial reductions)",
                "single_thread_value": one["iters"] / one["seconds"], "host_cores": c["cores"],
                "cpu_model": c["model"]}
 

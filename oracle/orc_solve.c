@@ -161,6 +161,17 @@ int orc_get_threads(void) {
 #endif
 }
 
+/* Rows per dynamically scheduled chunk: about eight chunks per thread, so
+   that a product with few long rows (R x, k rows) is still shared out. */
+static int par_chunk(int rows) {
+#ifdef _OPENMP
+    const int c = rows / (8 * omp_get_max_threads());
+    return c > 1 ? c : 1;
+#else
+    return rows > 1 ? rows : 1;
+#endif
+}
+
 static void *xm(size_t n) { void *p = malloc(n ? n : 1); if (!p) abort(); return p; }
 static double *zeros(int n) { double *p = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); if (!p) abort(); return p; }
 
@@ -168,7 +179,8 @@ static double *zeros(int n) { double *p = (double *)calloc((size_t)(n > 0 ? n : 
    out among threads, each row summed in the same order as serially, so the
    result does not depend on the thread count. */
 static void spmv(const orc_csr *M, const double *x, double *y) {
-#pragma omp parallel for schedule(dynamic, 64) if (M->nnz > 200000)
+    const int chunk = par_chunk(M->rows);
+#pragma omp parallel for schedule(dynamic, chunk) if (M->nnz > 200000)
     for (int r = 0; r < M->rows; ++r) {
         double s = 0.0;
         for (int e = M->rp[r]; e < M->rp[r + 1]; ++e) s += M->v[e] * x[M->ci[e]];
@@ -255,7 +267,8 @@ static void obj_product(orc_solver_t *s, const double *x) {
         spmv(&w->Q, x, s->qx);
         spmv(&w->R, x, s->rx);
         apply_middle(s);
-#pragma omp parallel for schedule(dynamic, 64) if (s->Rt.nnz > 200000)
+        const int chunk = par_chunk(n);
+#pragma omp parallel for schedule(dynamic, chunk) if (s->Rt.nnz > 200000)
         for (int r = 0; r < n; ++r) {
             double acc = 0.0;
             for (int e = s->Rt.rp[r]; e < s->Rt.rp[r + 1]; ++e) acc += s->Rt.v[e] * s->rx[s->Rt.ci[e]];
@@ -575,6 +588,7 @@ static void bb_prox(orc_solver_t *s, double tau) {
     }
     if (s->precond) a0 *= s->tol_scale * s->tol_scale;
     obj_product(s, s->x);
+#pragma omp parallel for schedule(static) if (n > 100000)
     for (int i = 0; i < n; ++i) {
         double g = s->qx[i] + w->c[i] - s->aty[i];
         double nx = s->precond ? s->x[i] - a0 * s->minv[i] * g : s->x[i] - a0 * g;
@@ -598,6 +612,7 @@ static void bb_prox(orc_solver_t *s, double tau) {
         }
         if (nrm <= s->inner_tol) break;
         obj_product(s, s->xp);
+#pragma omp parallel for schedule(static) if (n > 100000)
         for (int i = 0; i < n; ++i) {
             double last = s->grad[i];
             double cur = s->qx[i] + w->c[i] - s->aty[i] + inv * (s->xp[i] - s->x[i]);
@@ -607,6 +622,7 @@ static void bb_prox(orc_solver_t *s, double tau) {
         double tmp = dot(n, s->dirn, s->pbuf);
         if (s->snap) memcpy(s->snap, s->xp, (size_t)n * sizeof(double));
         double alpha = s->precond ? stms / tmp : (nrm * nrm) / tmp;
+#pragma omp parallel for schedule(static) if (n > 100000)
         for (int i = 0; i < n; ++i) {
             double cur = s->xp[i];
             double nx = s->precond ? cur - alpha * s->minv[i] * s->grad[i] : cur - alpha * s->grad[i];
