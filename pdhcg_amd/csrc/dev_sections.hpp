@@ -457,4 +457,287 @@ __global__ void __launch_bounds__(kBlock) section_residual_kernel(double *dres, 
     for (int i = 0; i < len; ++i) dres[st + i] = fixed[st + i] ? 0.0 : ((double)x[st + i] - p[i]) * d[st + i];
 }
 
+// ---- long SOC / RSOC sections: one workgroup per block ----
+//
+// Blocks above 64 vector slots with fixed slots run sec_soc / sec_rsoc with
+// the whole workgroup: the per-slot arrays live in a global scratch area,
+// every sum is a fixed-order workgroup reduction broadcast to all threads
+// (so each root finder takes the same steps in every thread), and barriers
+// separate a single thread's writes from the reads that follow them.
+
+// Workgroup sum and maximum; every thread receives the result.
+struct SecWg {
+    double lds[16];
+    double bc;
+    __device__ double sum(double v) {
+        double t[1] = {v};
+        block_reduce<1, 0>(t, lds);
+        if (threadIdx.x == 0) bc = t[0];
+        __syncthreads();
+        const double r = bc;
+        __syncthreads();
+        return r;
+    }
+    __device__ double max(double v) {
+        double t[1] = {v};
+        block_reduce<0, 1>(t, lds);
+        if (threadIdx.x == 0) bc = t[0];
+        __syncthreads();
+        const double r = bc;
+        __syncthreads();
+        return r;
+    }
+};
+
+__device__ inline double secw_shrunk_norm2(const double *a, const double *w, const char *fx, int n, double l,
+                                           SecWg &g) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (!fx[i]) {
+            const double u = w[i] * a[i] / (w[i] + l);
+            s += u * u;
+        }
+    return g.sum(s);
+}
+
+__device__ inline void secw_ball(const double *a, const double *w, const char *fx, int n, double r2, double *out,
+                                 SecWg &g) {
+    double f2 = 0.0, p2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (!fx[i]) {
+            f2 += a[i] * a[i];
+            p2 += (w[i] * a[i]) * (w[i] * a[i]);
+            out[i] = a[i];
+        }
+    f2 = g.sum(f2);
+    p2 = g.sum(p2);
+    if (f2 <= r2) return;
+    if (!(r2 > 0.0)) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (!fx[i]) out[i] = 0.0;
+        __syncthreads();
+        return;
+    }
+    const double hi = sqrt(p2) / sqrt(r2) * (1.0 + 1e-12) + DBL_MIN;
+    const double l =
+        sec_root_increasing([&](double t) { return r2 - secw_shrunk_norm2(a, w, fx, n, t, g); }, 0.0, hi);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (!fx[i]) out[i] = w[i] * a[i] / (w[i] + l);
+    __syncthreads();
+}
+
+// sec_soc_free_z by the workgroup; out's free slots are written by the thread
+// that owns them (i = threadIdx.x mod blockDim.x), the returned z is uniform.
+__device__ inline double secw_soc_free_z(const double *a, const double *w, const char *fx, int n, double az,
+                                         double wz, double *out, SecWg &g) {
+    double phi = 0.0, f2 = 0.0, p2 = 0.0, nf = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (fx[i]) phi += a[i] * a[i];
+        else {
+            f2 += a[i] * a[i];
+            p2 += (w[i] * a[i]) * (w[i] * a[i]);
+            out[i] = a[i];
+            nf += 1.0;
+        }
+    }
+    phi = g.sum(phi);
+    f2 = g.sum(f2);
+    p2 = g.sum(p2);
+    nf = g.sum(nf);
+    if (az >= 0.0 && phi + f2 <= az * az) return az;
+    if (nf == 0.0) return fmax(az, sqrt(phi));
+    if (phi == 0.0 && -wz * az >= sqrt(p2)) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (!fx[i]) out[i] = 0.0;
+        __syncthreads();
+        return 0.0;
+    }
+    auto neg = [&](double l) { return az - sqrt(phi + secw_shrunk_norm2(a, w, fx, n, l, g)) * (1.0 - l / wz); };
+    double l;
+    if (az == 0.0) l = wz;
+    else if (az > 0.0) l = sec_root_increasing(neg, 0.0, wz);
+    else {
+        double lo = wz, hi = 2.0 * wz;
+        for (int it = 0; it < 2100 && neg(hi) <= 0.0; ++it) {
+            lo = hi;
+            hi *= 2.0;
+        }
+        l = sec_root_increasing(neg, lo, hi);
+    }
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (!fx[i]) {
+            out[i] = w[i] * a[i] / (w[i] + l);
+            s += out[i] * out[i];
+        }
+    return sqrt(phi + g.sum(s));
+}
+
+__device__ inline void secw_soc(const double *a, const double *w, const char *fx, int vd, double *out, SecWg &g) {
+    const int n = vd + 1;
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) out[i] = a[i];
+    if (fx[n]) {
+        double phi = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (fx[i]) phi += a[i] * a[i];
+        phi = g.sum(phi);
+        secw_ball(a, w, fx, n, fmax(0.0, a[n] * a[n] - phi), out, g);
+    } else {
+        const double z = secw_soc_free_z(a, w, fx, n, a[n], w[n], out, g);
+        __syncthreads();
+        if (threadIdx.x == 0) out[n] = z;
+    }
+    __syncthreads();
+}
+
+// a and w are overwritten when neither end is fixed (rotated in place).
+__device__ inline void secw_rsoc(double *a, double *w, const char *fx, int vd, double *out, SecWg &g) {
+    const int si = vd, ti = vd + 1;
+    for (int i = threadIdx.x; i < vd + 2; i += blockDim.x) out[i] = a[i];
+    double phi = 0.0, f2 = 0.0, nf = 0.0;
+    for (int i = threadIdx.x; i < vd; i += blockDim.x) {
+        if (fx[i]) phi += a[i] * a[i];
+        else {
+            f2 += a[i] * a[i];
+            nf += 1.0;
+        }
+    }
+    phi = g.sum(phi);
+    f2 = g.sum(f2);
+    nf = g.sum(nf);
+    if (fx[si] && fx[ti]) {
+        secw_ball(a, w, fx, vd, fmax(0.0, 2.0 * a[si] * a[ti] - phi), out, g);
+        return;
+    }
+    if (fx[si] || fx[ti]) {
+        const int e = fx[si] ? si : ti, f = fx[si] ? ti : si;
+        const double E = a[e], af = a[f], wf = w[f];
+        if (!(E > 0.0)) {
+            for (int i = threadIdx.x; i < vd; i += blockDim.x)
+                if (!fx[i]) out[i] = 0.0;
+            __syncthreads();
+            if (threadIdx.x == 0) out[f] = fmax(af, 0.0);
+            __syncthreads();
+            return;
+        }
+        if (af >= 0.0 && phi + f2 <= 2.0 * E * af) return;
+        if (nf == 0.0) {
+            if (threadIdx.x == 0) out[f] = fmax(af, phi / (2.0 * E));
+            __syncthreads();
+            return;
+        }
+        const double hi = wf * (phi + f2 - 2.0 * E * af) / (2.0 * E * E) * (1.0 + 1e-12) + DBL_MIN;
+        const double l = sec_root_increasing(
+            [&](double t) { return 2.0 * E * (af + t * E / wf) - (phi + secw_shrunk_norm2(a, w, fx, vd, t, g)); },
+            0.0, hi);
+        for (int i = threadIdx.x; i < vd; i += blockDim.x)
+            if (!fx[i]) out[i] = w[i] * a[i] / (w[i] + l);
+        __syncthreads();
+        if (threadIdx.x == 0) out[f] = af + l * E / wf;
+        __syncthreads();
+        return;
+    }
+    if (a[si] >= 0.0 && a[ti] >= 0.0 && phi + f2 <= 2.0 * a[si] * a[ti]) return;
+    const double ss = sqrt(w[si]), st = sqrt(w[ti]), r = ss * st, sr = sqrt(r);
+    const double sp = ss * a[si], tp = st * a[ti];
+    __syncthreads();  // every thread has read a[si], a[ti], w[si], w[ti]
+    for (int i = threadIdx.x; i < vd; i += blockDim.x) {
+        a[i] = sr * a[i];
+        w[i] = w[i] / r;
+    }
+    if (threadIdx.x == 0) {
+        a[vd] = (sp - tp) * kSqrtHalf;
+        a[vd + 1] = (sp + tp) * kSqrtHalf;
+        w[vd] = w[vd + 1] = 1.0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < vd + 2; i += blockDim.x) out[i] = a[i];
+    const double z = secw_soc_free_z(a, w, fx, vd + 1, a[vd + 1], 1.0, out, g);
+    __syncthreads();
+    const double wr = out[vd];
+    for (int i = threadIdx.x; i < vd; i += blockDim.x)
+        if (!fx[i]) out[i] = out[i] / sr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[si] = fmax(0.0, (z + wr) * kSqrtHalf) / ss;
+        out[ti] = fmax(0.0, (z - wr) * kSqrtHalf) / st;
+    }
+    __syncthreads();
+}
+
+// sec_project_block for a long SOC/RSOC block by the workgroup: p (global,
+// vd + 2 values) is projected in place on its free slots; a, w, o are
+// scratch areas of vd + 2 values each.
+template <class X>
+__device__ void secw_project_block(double *p, int type, int vd, const double *d, const X *q, double tau,
+                                   const char *fx, double *a, double *w, double *o, SecWg &g) {
+    const int len = vd + 2;
+    double smax = 0.0, wmax = 0.0;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        a[i] = p[i] / d[i];
+        double m = 1.0;
+        if (q) {
+            m = 1.0 + tau * (double)q[i];
+            if (!(m > 0.0)) m = 1.0;
+        }
+        w[i] = fmax(m * d[i] * d[i], DBL_MIN);
+        smax = fmax(smax, fabs(a[i]));
+        if (!fx[i]) wmax = fmax(wmax, w[i]);
+    }
+    smax = g.max(smax);
+    wmax = g.max(wmax);
+    if (!(smax > 0.0) || !isfinite(smax)) return;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        a[i] /= smax;
+        if (wmax > 0.0) w[i] /= wmax;
+    }
+    __syncthreads();
+    if (type == CONE_STANDARD_SOC) secw_soc(a, w, fx, vd, o, g);
+    else secw_rsoc(a, w, fx, vd, o, g);
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x)
+        if (!fx[i]) p[i] = o[i] * smax * d[i];
+    __syncthreads();
+}
+
+// section_project_kernel for long blocks: one workgroup per block, scratch
+// holds 4 (vd + 2) values per block from off[b].
+template <class X>
+__global__ void __launch_bounds__(kBlock) section_long_project_kernel(X *v, const double *d, const X *q, double tau,
+                                                                       const char *fixed, SectionList S,
+                                                                       double *scratch, const long long *off,
+                                                                       const int *gate) {
+    if (gate && *gate == 0) return;
+    __shared__ SecWg g;
+    const int b = blockIdx.x;
+    const int st = S.start[b], t = S.type[b], vd = S.vdim[b], len = vd + 2;
+    double *p = scratch + off[b], *a = p + len, *w = a + len, *o = w + len;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) p[i] = (double)v[st + i];
+    __syncthreads();
+    secw_project_block<X>(p, t, vd, d + st, q ? q + st : nullptr, tau, fixed + st, a, w, o, g);
+    for (int i = threadIdx.x; i < len; i += blockDim.x)
+        if (!fixed[st + i]) v[st + i] = (X)p[i];
+}
+
+// section_residual_kernel for long blocks (layout as section_long_project_kernel).
+template <class X>
+__global__ void __launch_bounds__(kBlock) section_long_residual_kernel(double *dres, const X *x, const X *c,
+                                                                        const X *qx, const X *aty, const double *d,
+                                                                        const char *fixed, SectionList S,
+                                                                        double *scratch, const long long *off) {
+    __shared__ SecWg g;
+    const int b = blockIdx.x;
+    const int st = S.start[b], t = S.type[b], vd = S.vdim[b], len = vd + 2;
+    double *p = scratch + off[b], *a = p + len, *w = a + len, *o = w + len;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const double xi = (double)x[st + i];
+        const double rc = (double)c[st + i] + (qx ? (double)qx[st + i] : 0.0) - (double)aty[st + i];
+        p[i] = fixed[st + i] ? xi : xi - rc;
+    }
+    __syncthreads();
+    secw_project_block<X>(p, t, vd, d + st, (const X *)nullptr, 0.0, fixed + st, a, w, o, g);
+    for (int i = threadIdx.x; i < len; i += blockDim.x)
+        dres[st + i] = fixed[st + i] ? 0.0 : ((double)x[st + i] - p[i]) * d[st + i];
+}
+
 }  // namespace pdhcg

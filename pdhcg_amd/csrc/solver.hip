@@ -116,7 +116,10 @@ class DevConeSet {
     ConeList lrsoc, lsoc;                        // SOC/RSOC blocks above kMaxThreadCone vector slots
     ConeList powall;                             // every power block, for the primal violation
     SectionList sec;                             // blocks with fixed slots (any non-PSD type)
-    char *fixed = nullptr;                       // per-slot fixed flags, when sec.count > 0
+    SectionList lsec;                            // SOC/RSOC blocks with fixed slots above kMaxThreadCone
+    double *lsec_scratch = nullptr;              // 4 (v_dim + 2) values per long section block
+    long long *lsec_off = nullptr;
+    char *fixed = nullptr;                       // per-slot fixed flags, when sections exist
     std::vector<PsdGroup> groups;
     char *mask = nullptr;
     int total = 0;                               // all blocks, in input order:
@@ -137,8 +140,10 @@ class DevConeSet {
         std::vector<int> hs, hl;
         if (b->is_fixed && b->fixed_mask_size < n)
             fail("the fixed-slot mask has %d entries; %d variables", b->fixed_mask_size, n);
-        std::vector<int> sec_start, sec_vdim, sec_type, pall_start;
+        std::vector<int> sec_start, sec_vdim, sec_type, pall_start, lsec_start, lsec_vdim, lsec_type;
         std::vector<double> sec_alpha, pall_alpha;
+        std::vector<long long> lsec_off_h;
+        long long lsec_total = 0;
         for (int k = 0; k < b->num_cones; ++k) {
             const int st = b->start_idx[k], vd = b->v_dim[k], len = cone_block_slots(b, k);
             hs.push_back(st);
@@ -159,9 +164,14 @@ class DevConeSet {
             bool has_fixed = false;
             for (int i = st; b->is_fixed && i < st + len; ++i) has_fixed |= b->is_fixed[i] != 0;
             if (has_fixed) {
-                if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone)
-                    fail("second-order cone block %d with fixed slots has %d vector slots; at most %d are supported", k,
-                         vd, kMaxThreadCone);
+                if ((t == CONE_STANDARD_SOC || t == CONE_ROTATED_SOC) && vd > kMaxThreadCone) {
+                    lsec_start.push_back(st);
+                    lsec_vdim.push_back(vd);
+                    lsec_type.push_back((int)t);
+                    lsec_off_h.push_back(lsec_total);
+                    lsec_total += 4LL * (vd + 2);
+                    continue;
+                }
                 sec_start.push_back(st);
                 sec_vdim.push_back(vd);
                 sec_type.push_back((int)t);
@@ -200,6 +210,18 @@ class DevConeSet {
             sec.vdim = dupload<int>(sec_vdim, sec_vdim.size());
             sec.type = dupload<int>(sec_type, sec_type.size());
             sec.alpha = dupload<double>(sec_alpha, sec_alpha.size());
+        }
+        if (!lsec_start.empty()) {
+            lsec.count = (int)lsec_start.size();
+            lsec.start = dupload<int>(lsec_start, lsec_start.size());
+            lsec.vdim = dupload<int>(lsec_vdim, lsec_vdim.size());
+            lsec.type = dupload<int>(lsec_type, lsec_type.size());
+            std::vector<double> zero_alpha(lsec_start.size(), 0.0);
+            lsec.alpha = dupload<double>(zero_alpha, zero_alpha.size());
+            lsec_off = dupload<long long>(lsec_off_h, lsec_off_h.size());
+            lsec_scratch = dalloc<double>((size_t)lsec_total);
+        }
+        if (!sec_start.empty() || !lsec_start.empty()) {
             std::vector<char> hf(b->is_fixed, b->is_fixed + n);
             fixed = dupload<char>(hf, hf.size());
         }
@@ -249,6 +271,9 @@ class DevConeSet {
         if (sec.count)
             hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, (const X *)nullptr,
                                0.0, fixed, sec, gate);
+        if (lsec.count)
+            hipLaunchKernelGGL(section_long_project_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, v, d,
+                               (const X *)nullptr, 0.0, fixed, lsec, lsec_scratch, lsec_off, gate);
         for (const PsdGroup &g : groups)
             hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, v, g.starts, g.order, g.work,
                                gate);
@@ -278,6 +303,9 @@ class DevConeSet {
         if (sec.count)
             hipLaunchKernelGGL(section_project_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, v, d, q, tau, fixed,
                                sec, (const int *)nullptr);
+        if (lsec.count)
+            hipLaunchKernelGGL(section_long_project_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, v, d, q, tau,
+                               fixed, lsec, lsec_scratch, lsec_off, (const int *)nullptr);
     }
 
     // Dual-cone residuals (rc - P_{K*}(rc)) d of rc = c + qx - aty (qx may be
@@ -290,6 +318,9 @@ class DevConeSet {
         if (sec.count)
             hipLaunchKernelGGL(section_residual_kernel<X>, blocks(sec.count), dim3(kBlock), 0, st, dres, x, c, qx, aty,
                                d, fixed, sec);
+        if (lsec.count)
+            hipLaunchKernelGGL(section_long_residual_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, dres, x, c, qx,
+                               aty, d, fixed, lsec, lsec_scratch, lsec_off);
         if (rsoc.count)
             hipLaunchKernelGGL((cone_dual_residual_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0,
                                st, dres, c, qx, aty, d, rsoc);
@@ -326,9 +357,14 @@ class DevConeSet {
                 if (p) (void)hipFree(p);
             *L = ConeList();
         }
-        for (void *p : {(void *)sec.start, (void *)sec.vdim, (void *)sec.type, (void *)sec.alpha, (void *)fixed})
+        for (void *p : {(void *)sec.start, (void *)sec.vdim, (void *)sec.type, (void *)sec.alpha, (void *)fixed,
+                        (void *)lsec.start, (void *)lsec.vdim, (void *)lsec.type, (void *)lsec.alpha,
+                        (void *)lsec_scratch, (void *)lsec_off})
             if (p) (void)hipFree(p);
         sec = SectionList();
+        lsec = SectionList();
+        lsec_scratch = nullptr;
+        lsec_off = nullptr;
         fixed = nullptr;
         for (PsdGroup &g : groups) {
             if (g.starts) (void)hipFree(g.starts);
@@ -1780,7 +1816,7 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
         HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         DevConeSet set;
         set.build(cones, n);
-        if (dual && set.sec.count)
+        if (dual && (set.sec.count || set.lsec.count))
             throw std::runtime_error("blocks with fixed slots have no dual-cone residual without a primal point");
         std::vector<double> dh((size_t)n, 1.0);
         if (d) std::copy(d, d + n, dh.begin());

@@ -28,16 +28,16 @@ def _len(t, vd):
     return vd + 2 if t in (SOC, RSOC) else 3
 
 
-@pytest.mark.parametrize("t", [SOC, RSOC, EXP, POW])
-def test_section_projection_matches_oracle(t):
-    rng = np.random.default_rng(40 + t)
+def _section_case(t, rng, sizes, count, end_patterns=False):
     specs, fixed, x, d, blocks, pos = [], [], [], [], [], 0
-    for k in range(120):
-        vd = int(rng.choice([1, 2, 5, 17, 64])) if t in (SOC, RSOC) else 1
+    for k in range(count):
+        vd = int(rng.choice(sizes)) if t in (SOC, RSOC) else 1
         L = _len(t, vd)
         alpha = float(rng.uniform(0.1, 0.9)) if t == POW else 0.0
         anchor = cone_point(t, vd, alpha, rng)
-        fx = (rng.uniform(size=L) < 0.4).astype(np.uint8)
+        fx = (rng.uniform(size=L) < rng.choice([0.05, 0.4, 0.9])).astype(np.uint8)
+        if end_patterns:  # cycle through the (w/s, z/t) fixing patterns
+            fx[vd], fx[vd + 1] = k % 4 // 2, k % 2
         if not fx.any():  # blocks without fixed slots take the plain projection
             fx[int(rng.integers(L))] = 1
         if fx.all():
@@ -52,7 +52,10 @@ def test_section_projection_matches_oracle(t):
         x += list(a * dd) + [rng.normal()]
         d += list(dd) + [1.0]
         pos += L + 1
-    x, d = np.array(x), np.array(d)
+    return specs, fixed, np.array(x), np.array(d), blocks
+
+
+def _check_sections(t, specs, fixed, x, d, blocks):
     got = pdhcg_amd.project_cones(specs, x, d, fixed=fixed)
     for pos, vd, alpha, fx, a, dd in blocks:
         L = _len(t, vd)
@@ -64,6 +67,19 @@ def test_section_projection_matches_oracle(t):
     # slots outside the blocks are untouched
     gaps = [p + _len(t, vd) for p, vd, *_ in blocks]
     assert np.all(got[gaps] == x[gaps])
+
+
+@pytest.mark.parametrize("t", [SOC, RSOC, EXP, POW])
+def test_section_projection_matches_oracle(t):
+    rng = np.random.default_rng(40 + t)
+    _check_sections(t, *_section_case(t, rng, [1, 2, 5, 17, 64], 120))
+
+
+@pytest.mark.parametrize("t", [SOC, RSOC])
+def test_long_section_projection_matches_oracle(t):
+    # blocks above 64 vector slots: one workgroup per block
+    rng = np.random.default_rng(80 + t)
+    _check_sections(t, *_section_case(t, rng, [65, 130, 700, 3000], 32, end_patterns=True))
 
 
 @pytest.mark.parametrize("alpha,inp,w,fx,exp,tol", [
@@ -124,7 +140,7 @@ def test_power_sharp_fixed_axis():
 
 
 @pytest.mark.parametrize("norm", [capi.NORM_TYPE_L_INF, capi.NORM_TYPE_L2])
-@pytest.mark.parametrize("v_dim", [1, 32, 64])
+@pytest.mark.parametrize("v_dim", [1, 32, 64, 32768])
 def test_zero_w_soc_optimal_at_start(norm, v_dim):
     g = pdhcg_amd.solve(fc.zero_w_soc(v_dim), optimality_norm=norm, **fc.ONE_STEP)
     assert g.status == "OPTIMAL" and g.iterations == 0, (g.status, g.iterations)
