@@ -128,22 +128,41 @@ struct Red {
     }
 };
 
-// y = M x for a CSR matrix of `rows` rows.
+// y = M x for a CSR matrix of `rows` rows.  With one workgroup per problem
+// and two waves per SIMD there is little occupancy to hide memory latency,
+// so each lane keeps four independent partial sums (four loads in flight)
+// and adds them in a fixed order.
 __device__ inline void spmv(const int *rp, const int *ci, const double *v, const double *x, double *y, int rows,
                             int wave_rows) {
     if (wave_rows) {
         const int lane = threadIdx.x % kWave;
         for (int r = threadIdx.x / kWave; r < rows; r += kBW) {
-            double s = 0.0;
-            for (int e = rp[r] + lane; e < rp[r + 1]; e += kWave) s += v[e] * x[ci[e]];
-            s = wave_sum(s);
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            const int end = rp[r + 1];
+            int e = rp[r] + lane;
+            for (; e + 3 * kWave < end; e += 4 * kWave) {
+                s0 += v[e] * x[ci[e]];
+                s1 += v[e + kWave] * x[ci[e + kWave]];
+                s2 += v[e + 2 * kWave] * x[ci[e + 2 * kWave]];
+                s3 += v[e + 3 * kWave] * x[ci[e + 3 * kWave]];
+            }
+            for (; e < end; e += kWave) s0 += v[e] * x[ci[e]];
+            const double s = wave_sum((s0 + s1) + (s2 + s3));
             if (lane == 0) y[r] = s;
         }
     } else {
         for (int r = threadIdx.x; r < rows; r += kBT) {
-            double s = 0.0;
-            for (int e = rp[r]; e < rp[r + 1]; ++e) s += v[e] * x[ci[e]];
-            y[r] = s;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            const int end = rp[r + 1];
+            int e = rp[r];
+            for (; e + 3 < end; e += 4) {
+                s0 += v[e] * x[ci[e]];
+                s1 += v[e + 1] * x[ci[e + 1]];
+                s2 += v[e + 2] * x[ci[e + 2]];
+                s3 += v[e + 3] * x[ci[e + 3]];
+            }
+            for (; e < end; ++e) s0 += v[e] * x[ci[e]];
+            y[r] = (s0 + s1) + (s2 + s3);
         }
     }
     __syncthreads();
@@ -184,9 +203,16 @@ struct Ctx {
         const int lane = threadIdx.x % kWave;
         for (int r = threadIdx.x / kWave; r < k; r += kBW) {
             const double *row = R + (long long)r * n;
-            double s = 0.0;
-            for (int j = lane; j < n; j += kWave) s += row[j] * x[j];
-            s = wave_sum(s);
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four loads in flight, fixed order
+            int j = lane;
+            for (; j + 3 * kWave < n; j += 4 * kWave) {
+                s0 += row[j] * x[j];
+                s1 += row[j + kWave] * x[j + kWave];
+                s2 += row[j + 2 * kWave] * x[j + 2 * kWave];
+                s3 += row[j + 3 * kWave] * x[j + 3 * kWave];
+            }
+            for (; j < n; j += kWave) s0 += row[j] * x[j];
+            const double s = wave_sum((s0 + s1) + (s2 + s3));
             if (lane == 0) rx[r] = s;
         }
         __syncthreads();
@@ -208,9 +234,16 @@ struct Ctx {
         }
         __syncthreads();
         for (int j = threadIdx.x; j < n; j += kBT) {
-            double s = qd[j] * x[j];
-            for (int r = 0; r < k; ++r) s += R[(long long)r * n + j] * rb[r];
-            qx[j] = s;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four loads in flight, fixed order
+            int r = 0;
+            for (; r + 3 < k; r += 4) {
+                s0 += R[(long long)r * n + j] * rb[r];
+                s1 += R[(long long)(r + 1) * n + j] * rb[r + 1];
+                s2 += R[(long long)(r + 2) * n + j] * rb[r + 2];
+                s3 += R[(long long)(r + 3) * n + j] * rb[r + 3];
+            }
+            for (; r < k; ++r) s0 += R[(long long)r * n + j] * rb[r];
+            qx[j] = qd[j] * x[j] + ((s0 + s1) + (s2 + s3));
         }
         __syncthreads();
     }

@@ -35,8 +35,10 @@ def _agree(p, g, o, eps=EPS, affine=False):
 
 def test_batch_of_mixed_kinds_matches_oracle():
     # uneven sizes; each case converges for the oracle in a few thousand steps
+    # (lowrank_dense_d seeds 1, 3 and 5 at n = 60 do not: their inner solve
+    # runs to its body limit at every step, here and in the oracle)
     cases = [("lp", 40, 0), ("lp", 51, 1), ("diag", 77, 0), ("diag", 88, 1), ("lowrank", 80, 1), ("lowrank", 114, 0),
-             ("lowrank_diag", 151, 0), ("lowrank_diag", 162, 1), ("lowrank_dense_d", 60, 1),
+             ("lowrank_diag", 151, 0), ("lowrank_diag", 162, 1), ("lowrank_dense_d", 60, 2),
              ("lowrank_dense_d", 70, 0), ("lp", 150, 3), ("lowrank_diag", 200, 2)]
     problems = [workloads.random_problem(kind, seed=seed, n=n, m=max(10, n // 3)) for kind, n, seed in cases]
     sols = pdhcg_amd.solve_batch(problems, eps=EPS, iteration_limit=200000, time_limit=60.0)
