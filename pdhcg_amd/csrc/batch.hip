@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -244,11 +245,31 @@ bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const
     return true;
 }
 
-template <class T> T *upload(const std::vector<T> &h) {
+template <class T> T *upload(const T *h, size_t count) {
     T *d = nullptr;
-    HIP_OK(hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
-    if (!h.empty()) HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&d, std::max<size_t>(count, 1) * sizeof(T)));
+    if (count) HIP_OK(hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice));
     return d;
+}
+template <class T> T *upload(const std::vector<T> &h) { return upload(h.data(), h.size()); }
+
+// Host buffer of `count` elements left uninitialised (every element is
+// written before it is read).
+template <class T> std::unique_ptr<T[]> raw_buffer(size_t count) {
+    return std::unique_ptr<T[]>(new T[std::max<size_t>(count, 1)]);
+}
+
+// Runs f(i) for i < count on up to 16 host threads.
+template <class F> void parallel_for(int count, F f) {
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next++; i < count; i = next++) f(i);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
 }
 
 }  // namespace
@@ -276,24 +297,14 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         std::vector<Packed> all((size_t)count);
         std::vector<const char *> why((size_t)count, nullptr);
         std::vector<std::string> err((size_t)count);
-        {
-            const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-            std::atomic<int> next{0};
-            auto work = [&]() {
-                for (int i = next++; i < count; i = next++) {
-                    try {
-                        why[i] = unsupported(problems[i]);
-                        if (!why[i]) pack(problems[i], P, all[i], &why[i]);
-                    } catch (const std::exception &e) {
-                        err[i] = e.what();
-                    }
-                }
-            };
-            std::vector<std::thread> pool;
-            for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-            work();
-            for (std::thread &t : pool) t.join();
-        }
+        parallel_for(count, [&](int i) {
+            try {
+                why[i] = unsupported(problems[i]);
+                if (!why[i]) pack(problems[i], P, all[i], &why[i]);
+            } catch (const std::exception &e) {
+                err[i] = e.what();
+            }
+        });
         for (int i = 0; i < count; ++i)
             if (!err[i].empty()) fail("problem %d: %s", i, err[i].c_str());
         // largest problems first, so they start in the first round of workgroups
@@ -309,17 +320,28 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         for (int i : idx) pk.push_back(std::move(all[i]));
         const int B = (int)pk.size();
         if (B == 0) return 0;
-        // pool the problems
+        // pool the problems: offsets first, then every problem copied into
+        // its slices of the pools in parallel
         std::vector<Inst> insts((size_t)B);
-        std::vector<int> ipool;
-        std::vector<double> vpool;
-        std::vector<Cone> cones;
         std::vector<HostState> states((size_t)B);
-        long long slab_total = 0;
+        std::vector<long long> ioff((size_t)B + 1, 0), voff((size_t)B + 1, 0), coff((size_t)B + 1, 0),
+            soff((size_t)B + 1, 0);
         for (int b = 0; b < B; ++b) {
-            Inst I = pk[b].inst;
-            const int io = (int)ipool.size();
-            const long long vo = (long long)vpool.size();
+            ioff[b + 1] = ioff[b] + (long long)pk[b].ai.size();
+            voff[b + 1] = voff[b] + (long long)pk[b].av.size();
+            coff[b + 1] = coff[b] + (long long)pk[b].cones.size();
+            soff[b + 1] = soff[b] + (long long)pk[b].slab.size();
+        }
+        if (ioff[B] > (long long)INT32_MAX) fail("the batch's index pool exceeds 2^31 entries");
+        auto ipool = raw_buffer<int>((size_t)ioff[B]);
+        auto vpool = raw_buffer<double>((size_t)voff[B]);
+        auto cones = raw_buffer<Cone>((size_t)coff[B]);
+        auto slabs = raw_buffer<double>((size_t)soff[B]);
+        parallel_for(B, [&](int b) {
+            Packed &q = pk[b];
+            Inst I = q.inst;
+            const int io = (int)ioff[b];
+            const long long vo = voff[b];
             I.a_rp += io;
             I.a_ci += io;
             I.at_rp += io;
@@ -328,19 +350,19 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             I.at_v += vo;
             I.rdense += vo;
             I.dmat += vo;
-            I.cone0 = (int)cones.size();
-            I.vec = slab_total;
-            slab_total += (long long)pk[b].slab.size();
-            ipool.insert(ipool.end(), pk[b].ai.begin(), pk[b].ai.end());
-            vpool.insert(vpool.end(), pk[b].av.begin(), pk[b].av.end());
-            cones.insert(cones.end(), pk[b].cones.begin(), pk[b].cones.end());
+            I.cone0 = (int)coff[b];
+            I.vec = soff[b];
+            std::copy(q.ai.begin(), q.ai.end(), ipool.get() + ioff[b]);
+            std::copy(q.av.begin(), q.av.end(), vpool.get() + vo);
+            std::copy(q.cones.begin(), q.cones.end(), cones.get() + coff[b]);
+            std::copy(q.slab.begin(), q.slab.end(), slabs.get() + soff[b]);
             insts[b] = I;
-            states[b] = pk[b].st;
-        }
-        if (ipool.size() > (size_t)INT32_MAX) fail("the batch's index pool exceeds 2^31 entries");
-        std::vector<double> slabs;
-        slabs.reserve((size_t)slab_total);
-        for (const Packed &p : pk) slabs.insert(slabs.end(), p.slab.begin(), p.slab.end());
+            states[b] = q.st;
+            // the pools hold the data now
+            std::vector<int>().swap(q.ai);
+            std::vector<double>().swap(q.av);
+            std::vector<double>().swap(q.slab);
+        });
         Params dp;
         dp.ef = std::max(1, P.termination_evaluation_frequency);
         dp.iteration_limit = P.termination_criteria.iteration_limit;
@@ -367,10 +389,10 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         };
         Params *d_params = (Params *)track(upload(std::vector<Params>{dp}));
         Inst *d_insts = (Inst *)track(upload(insts));
-        int *d_ipool = (int *)track(upload(ipool));
-        double *d_vpool = (double *)track(upload(vpool));
-        double *d_slabs = (double *)track(upload(slabs));
-        Cone *d_cones = (Cone *)track(upload(cones));
+        int *d_ipool = (int *)track(upload(ipool.get(), (size_t)ioff[B]));
+        double *d_vpool = (double *)track(upload(vpool.get(), (size_t)voff[B]));
+        double *d_slabs = (double *)track(upload(slabs.get(), (size_t)soff[B]));
+        Cone *d_cones = (Cone *)track(upload(cones.get(), (size_t)coff[B]));
         State *d_states = (State *)track(upload(states));
         Out *d_out = (Out *)track(upload(std::vector<Out>((size_t)B)));
         hipStream_t st;
@@ -403,13 +425,13 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             std::fprintf(stderr, "[batch] %d problems: host scaling and packing %.3fs, device %.3fs in %d windows\n", B,
                          std::chrono::duration<double>(t_dev - t0).count(),
                          std::chrono::duration<double>(t_end - t_dev).count(), windows);
-        HIP_OK(hipMemcpy(slabs.data(), d_slabs, slabs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(slabs.get(), d_slabs, (size_t)soff[B] * sizeof(double), hipMemcpyDeviceToHost));
         (void)hipStreamDestroy(st);
         for (int b = 0; b < B; ++b) {
             const Inst &I = insts[b];
             const ScaledProblem &sp = pk[b].sp;
             const Out &o = out[b];
-            const double *V = slabs.data() + I.vec;
+            const double *V = slabs.get() + I.vec;
             const double *xp = V + (size_t)vXP * I.n, *aty = V + (size_t)vATY * I.n, *qx = V + (size_t)vQX * I.n;
             const double *yp = V + (size_t)kNVec * I.n + (size_t)wYP * I.m;
             pdhcg_result_t *r = ccalloc<pdhcg_result_t>(1);
