@@ -355,11 +355,34 @@ def test_long_blocks_diag_q_e2e():
     _device_vs_oracle(p, 1e-6, iteration_limit=200000)
 
 
-def test_unsupported_cone_configurations_are_rejected():
-    # fixed slots in a second-order block longer than 64 vector slots
+def test_fixed_slot_in_long_block_closed_form():
+    # A second-order block of v_dim 65 (slots 0..65 the vector, slot 66 the
+    # bound) with the bound set to 1 by a row and slot 3 fixed at 0.5: the
+    # other 65 vector slots range over the ball of radius sqrt(1 - 0.25), so
+    # the optimum is c66 + 0.5 c3 - sqrt(0.75) ||c_rest||.
     n = 67
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal(n)
+    A = sp.csr_matrix(([1.0], ([0], [n - 1])), shape=(1, n))
+    p = QPData(c=c, A=A, con_lb=[1.0], con_ub=[1.0], var_lb=[-1e30] * n, var_ub=[1e30] * n,
+               cones=[(SOC, 0, 65)], cone_fixed=[(0, 3, 0.5)])
+    rest = np.delete(c[:n - 1], 3)
+    exact = c[n - 1] + 0.5 * c[3] - np.sqrt(0.75) * np.linalg.norm(rest)
+    eps = 1e-7
+    g = pdhcg_amd.solve(p, eps=eps, iteration_limit=200000)
+    o = oracle_solve(p, eps=eps, iteration_limit=200000)
+    assert g.status == o.status == "OPTIMAL", (g.status, o.status)
+    assert g.x[3] == 0.5
+    assert abs(g.pobj - exact) <= 20 * eps * (1 + abs(exact)), (g.pobj, exact)
+    assert abs(g.pobj - o.pobj) <= 20 * eps * (1 + abs(exact)), (g.pobj, o.pobj)
+
+
+def test_fixed_psd_section_is_rejected():
+    # fixed slots in a PSD block are refused, as in the reference
+    # (src/pdhcg.c:632-636)
+    n = 6
     p = QPData(c=np.zeros(n), A=sp.csr_matrix(np.eye(1, n)), con_lb=[1.0], con_ub=[1.0], var_lb=[-1e30] * n,
-               var_ub=[1e30] * n, cones=[(SOC, 0, 65)], cone_fixed=[(0, 3, 0.5)])
+               var_ub=[1e30] * n, cones=[(PSD, 0, 3)], cone_fixed=[(0, 1, 0.5)])
     assert pdhcg_amd.solve(p, eps=1e-6) is None
 
 
