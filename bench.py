@@ -288,6 +288,9 @@ def run_warm(train_count, test_count, eps):
         obj = max(abs(a.pobj - b.pobj) / (1 + abs(a.pobj)) for a, b in zip(cold, warm))
         rec = {"cold_batch_s": cold_s, "warm_batch_s": warm_s, "cold_iterations_mean": float(ic.mean()),
                "warm_iterations_mean": float(iw.mean()), "median_iteration_ratio": float(np.median(iw / ic)),
+               # all instances run at once, one per workgroup: the batch's
+               # wall time follows its slowest instance
+               "cold_iterations_max": int(ic.max()), "warm_iterations_max": int(iw.max()),
                "optimal": [s_.status for s_ in warm].count("OPTIMAL"), "max_rel_objective_diff": obj}
         if tol == eps:
             xs = np.stack([s_.x for s_ in cold])
