@@ -155,13 +155,18 @@ class WarmStart:
         T = self.net.predict(Z).astype(np.float64) * self.scale + self.mean
         return T[:, : self.n], T[:, self.n:]
 
-    def start(self, problem, z):
-        """A copy of ``problem`` (QPData) carrying the predicted start values."""
+    def start(self, problem, z, dual: bool = True):
+        """A copy of ``problem`` (QPData) carrying the predicted start values.
+
+        With ``dual=False`` only the primal start is set and the dual starts
+        at zero.  The dual prediction is the less accurate one; it lowers the
+        mean iteration count but, on a few instances, raises theirs above the
+        cold solve's, and a batch's wall time follows its slowest instance."""
         x, y = self.predict(np.asarray(z, dtype=np.float32).reshape(1, -1))
         lb = -np.inf if problem.var_lb is None else problem.var_lb
         ub = np.inf if problem.var_ub is None else problem.var_ub
         x = np.clip(x[0], lb, ub)
-        return replace(problem, primal_start=x, dual_start=y[0])
+        return replace(problem, primal_start=x, dual_start=y[0] if dual else None)
 
     def close(self):
         self.net.close()

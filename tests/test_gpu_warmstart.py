@@ -69,3 +69,16 @@ def test_warm_started_device_solve_matches_oracle(trained):
     o = oracle_solve(p, eps=EPS, iteration_limit=100000)
     assert g.status == o.status == "OPTIMAL"
     assert abs(g.pobj - o.pobj) <= 5 * EPS * (1 + abs(o.pobj))
+
+
+def test_primal_only_start_solves_agree_with_cold(trained):
+    fam, ws, Zte, pte = trained
+    warm_p = [ws.start(p, z, dual=False) for p, z in zip(pte, Zte)]
+    assert all(p.dual_start is None and p.primal_start is not None for p in warm_p)
+    cold = pdhcg_amd.solve_batch(pte, eps=EPS, iteration_limit=100000, time_limit=120.0)
+    warm = pdhcg_amd.solve_batch(warm_p, eps=EPS, iteration_limit=100000, time_limit=120.0)
+    for p, c, w in zip(pte, cold, warm):
+        assert c.status == w.status == "OPTIMAL"
+        assert abs(c.pobj - w.pobj) <= 5 * EPS * (1 + abs(c.pobj)), (c.pobj, w.pobj)
+        k = kkt(p, w.x, w.y)
+        assert k["rel_primal"] < 10 * EPS and k["rel_dual"] < 10 * EPS and k["rel_gap"] < 10 * EPS, k
