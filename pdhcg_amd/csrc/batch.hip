@@ -245,10 +245,18 @@ bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const
     return true;
 }
 
+// Device copy of h; on a failed copy the allocation is released before the
+// error propagates.
 template <class T> T *upload(const T *h, size_t count) {
     T *d = nullptr;
     HIP_OK(hipMalloc(&d, std::max<size_t>(count, 1) * sizeof(T)));
-    if (count) HIP_OK(hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice));
+    if (count) {
+        const hipError_t e = hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            HIP_OK(e);
+        }
+    }
     return d;
 }
 template <class T> T *upload(const std::vector<T> &h) { return upload(h.data(), h.size()); }

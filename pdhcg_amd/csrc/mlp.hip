@@ -735,8 +735,20 @@ int pdhcg_mfma_gemm(int ta, int tb, int M, int N, int K, const float *A, const f
     }
     return guarded("pdhcg_mfma_gemm", [&] {
         const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
-        float *fa = dnew<float>(na), *fb = dnew<float>(nb), *fc = dnew<float>(nc);
-        bf16 *ba = dnew<bf16>(na), *bb = dnew<bf16>(nb);
+        // released on every exit, including a HIP error part way through
+        std::vector<void *> held;
+        struct Release {
+            std::vector<void *> &h;
+            ~Release() {
+                for (void *p : h) (void)hipFree(p);
+            }
+        } release{held};
+        auto hold = [&](auto *p) {
+            held.push_back((void *)p);
+            return p;
+        };
+        float *fa = hold(dnew<float>(na)), *fb = hold(dnew<float>(nb)), *fc = hold(dnew<float>(nc));
+        bf16 *ba = hold(dnew<bf16>(na)), *bb = hold(dnew<bf16>(nb));
         HIP_OK(hipMemcpy(fa, A, na * sizeof(float), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(fb, B, nb * sizeof(float), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_grid((long long)na)), dim3(256), 0, 0, fa, ba, (long long)na);
@@ -749,7 +761,6 @@ int pdhcg_mfma_gemm(int ta, int tb, int M, int N, int K, const float *A, const f
         else if (ta && !tb) launch<true, false, kStore>(0, a);
         else launch<true, true, kStore>(0, a);
         HIP_OK(hipMemcpy(C, fc, nc * sizeof(float), hipMemcpyDeviceToHost));
-        for (void *p : {(void *)fa, (void *)fb, (void *)fc, (void *)ba, (void *)bb}) (void)hipFree(p);
         return 0;
     });
 }
