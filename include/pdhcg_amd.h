@@ -5,6 +5,10 @@
  * of reference src/solver.cu:35-210, stepped in chunks of iterations) so that
  * benchmarks and tests can time the iteration without setup, and they select
  * the arithmetic precision out of band (the parameter struct is ABI).
+ *
+ * The precision and graph-window switches are process-wide and unsynchronised:
+ * set them before solves start, not while another thread is solving.  Solves
+ * on separate handles share no device state and may run on separate threads.
  */
 #pragma once
 
