@@ -7,8 +7,9 @@
  * the arithmetic precision out of band (the parameter struct is ABI).
  *
  * The precision and graph-window switches are process-wide and unsynchronised:
- * set them before solves start, not while another thread is solving.  Solves
- * on separate handles share no device state and may run on separate threads.
+ * set them before solves start, not while another thread is solving.  Each
+ * solve owns its HIP stream and device buffers; a cancel request applies to
+ * every running solve.
  */
 #pragma once
 
