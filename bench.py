@@ -101,10 +101,15 @@ def run_gpu(p, precision, steps, warmup, eps, max_solve_s):
     st.close()
     # Q products in the timed steps: one per step at x_k plus one per BB body
     # that ran; together these are the inner-iteration count of the window
+    # the returned point checked by an independent numpy FP64 KKT recomputation
+    from tests.kkt import kkt
+    k = kkt(p, res.x, res.y)
     return dict(setup_s=setup_s, step_s=t1 - t0, kernel_times=kt, q_products=inner1 - inner0,
                 total_iters=res.iterations,
                 inner_iters=res.inner_iterations, status=res.status, time_to_tol=t_end - t_solve0,
-                rel=(res.rel_primal, res.rel_dual, res.rel_gap), pobj=res.pobj)
+                rel=(res.rel_primal, res.rel_dual, res.rel_gap), pobj=res.pobj,
+                numpy_kkt=(float(k["rel_primal"]), float(k["rel_dual"]), float(k["rel_gap"])),
+                numpy_pobj=float(k["pobj"]), x=res.x)
 
 
 def copy_probe(mib):
@@ -221,16 +226,78 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
     if with_cpu:
         from tests.oracle_lib import oracle_solve
         t = time.perf_counter()
-        done = 0
-        for p_ in problems[::64]:
-            oracle_solve(p_, eps=eps, iteration_limit=100000)
+        done, same_status, obj_diff = 0, 0, 0.0
+        for i in range(0, count, 64):
+            o = oracle_solve(problems[i], eps=eps, iteration_limit=100000)
             done += 1
+            g = sols[i]
+            if g is not None and o is not None and g.status == o.status:
+                same_status += 1
+                obj_diff = max(obj_diff, abs(g.pobj - o.pobj) / (1.0 + abs(o.pobj)))
             if time.perf_counter() - t > cpu_seconds:
                 break
         cpu_ms = 1e3 * (time.perf_counter() - t) / max(done, 1)
         out["cpu_baseline"] = {"ms_per_problem": cpu_ms, "cores": 1, "kind": "port",
                                "sample": f"FP64 oracle on {done} of every 64th instance to 1e-6, single thread"}
+        # the oracle's solves of those instances against the batch's results
+        out["oracle_check"] = {"instances": done, "same_status": same_status,
+                               "max_rel_objective_diff": obj_diff}
     return out
+
+
+def run_conic(seconds, eps, cpu_seconds, with_cpu):
+    """The conic mix (workloads.conic_mix at scale 1: n = 572,102 with SOC,
+    RSOC, 33,333 exponential, 16,667 power and 67 PSD blocks of order
+    U{16..128}, an affine SOC block F x + g with 2M nnz, A with 2M nnz, sparse
+    Q = B'B + diag): FP64 and mixed device solves toward eps under a time
+    limit, and the FP64 oracle's iteration rate on a bounded sample."""
+    import numpy as np
+    import pdhcg_amd
+    from pdhcg_amd import workloads
+    t = time.perf_counter()
+    p = workloads.conic_mix()
+    log(f"[bench] conic mix generated in {time.perf_counter() - t:.1f}s: n={p.c.size} m={p.A.shape[0]} "
+        f"affine rows={p.F.shape[0]} cone blocks={len(p.cones)}")
+    kinds = {}
+    for c in p.cones:
+        kinds[c[0]] = kinds.get(c[0], 0) + 1
+    out = {"workload": "conic mix: n=%d, %d SOC, %d RSOC, %d EXP, %d POW, %d PSD blocks (orders 16..128), "
+                       "A %dx%d nnz %d, affine SOC rows %d nnz(F) %d, sparse Q nnz %d"
+                       % (p.c.size, kinds.get(capi_const("CONE_STANDARD_SOC"), 0),
+                          kinds.get(capi_const("CONE_ROTATED_SOC"), 0), kinds.get(capi_const("CONE_EXPONENTIAL"), 0),
+                          kinds.get(capi_const("CONE_POWER"), 0), kinds.get(capi_const("CONE_PSD"), 0),
+                          p.A.shape[0], p.c.size, p.A.nnz, p.F.shape[0], p.F.nnz, p.Q.nnz),
+           "time_limit_s": seconds, "tol": eps}
+    for name, prec in (("fp64", pdhcg_amd.PRECISION_FP64), ("mixed", pdhcg_amd.PRECISION_MIXED)):
+        t = time.perf_counter()
+        st = pdhcg_amd.Stepper(p, prec, eps=eps, time_limit=seconds)
+        setup = time.perf_counter() - t
+        t = time.perf_counter()
+        rc = 0
+        while rc == 0:
+            rc = st.run(100)
+        st.synchronize()
+        wall = time.perf_counter() - t
+        r = st.result()
+        st.close()
+        out[name] = {"status": r.status, "iterations": r.iterations, "inner_iterations": r.inner_iterations,
+                     "seconds": wall, "setup_s": setup, "iter_per_s": r.iterations / wall,
+                     "rel_residuals": (r.rel_primal, r.rel_dual, r.rel_gap), "pobj": r.pobj}
+        log(f"[bench] conic {name}: {r.status} after {r.iterations} iterations ({r.inner_iterations} inner) "
+            f"in {wall:.1f}s, rel residuals {r.rel_primal:.2e} {r.rel_dual:.2e} {r.rel_gap:.2e}")
+    if with_cpu:
+        c = run_cpu_baseline(p, eps, cpu_seconds)
+        a = c["all"]
+        out["cpu_baseline"] = {"value": a["iters"] / a["seconds"], "unit": "iter/s", "cores": a["threads"],
+                               "kind": "port", "sample": f"FP64 oracle: {a['iters']} main-loop iterations in "
+                                                         f"{a['seconds']:.1f}s on {a['threads']} threads after "
+                                                         f"{c['setup_s']:.1f}s setup"}
+    return out
+
+
+def capi_const(name):
+    from pdhcg_amd import capi
+    return getattr(capi, name)
 
 
 def mlp_step_flops(in_dim, hidden, out_dim, batch):
@@ -350,6 +417,10 @@ def main():
     ap.add_argument("--warm-train", type=int, default=4096, metavar="N",
                     help="training instances of the learned warm start (0: skip it)")
     ap.add_argument("--warm-test", type=int, default=256, metavar="N")
+    ap.add_argument("--conic-seconds", type=float, default=40.0,
+                    help="time limit of each conic-mix solve (0: skip the conic mix)")
+    ap.add_argument("--conic-only", action="store_true",
+                    help="only run the conic-mix section (prints {\"conic_mix\": ...})")
     ap.add_argument("--warm-only", action="store_true",
                     help="only run the learned warm start section (prints {\"warm_start\": ...})")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -365,6 +436,10 @@ def main():
 
     if args.warm_only:
         print(json.dumps({"warm_start": run_warm(args.warm_train, args.warm_test, args.eps)}), flush=True)
+        return
+    if args.conic_only:
+        print(json.dumps({"conic_mix": run_conic(args.conic_seconds, args.eps, args.cpu_seconds, not args.no_cpu)}),
+              flush=True)
         return
 
     from pdhcg_amd import workloads
@@ -449,16 +524,31 @@ def main():
         "solve_iterations": main_run["total_iters"],
         "solve_inner_iterations": main_run["inner_iters"],
         "solve_rel_residuals": main_run["rel"],
+        "solve_pobj": main_run["pobj"],
+        "solve_numpy_fp64_kkt": main_run["numpy_kkt"],
         "kernels": kernels,
     }
     if args.compare:
+        import numpy as np
         other = [k for k in runs if k != args.precision][0]
         o = runs[other]
+        fp64 = runs["fp64"]
         out["other_precision"] = {"precision": other, "iter_per_s": args.steps / o["step_s"],
                                   "time_to_tol_s": o["time_to_tol"], "status": o["status"],
-                                  "iterations": o["total_iters"], "rel_residuals": o["rel"], "pobj": o["pobj"]}
+                                  "iterations": o["total_iters"], "rel_residuals": o["rel"], "pobj": o["pobj"],
+                                  # an independent numpy FP64 KKT recomputation of the returned point
+                                  "numpy_fp64_kkt": o["numpy_kkt"]}
+        # the mixed result against the FP64 solve of the same problem
+        mixed = runs["mixed"]
+        out["mixed_vs_fp64"] = {
+            "pobj_fp64": fp64["pobj"], "pobj_mixed": mixed["pobj"],
+            "rel_objective_diff": abs(mixed["pobj"] - fp64["pobj"]) / (1.0 + abs(fp64["pobj"])),
+            "rel_x_diff": float(np.linalg.norm(mixed["x"] - fp64["x"]) / max(np.linalg.norm(fp64["x"]), 1e-300)),
+            "speedup_time_to_tol": fp64["time_to_tol"] / mixed["time_to_tol"]}
     if args.batch > 0:
         out["batched"] = run_batch(args.batch, args.eps, args.cpu_seconds, not args.no_cpu)
+    if args.conic_seconds > 0:
+        out["conic_mix"] = run_conic(args.conic_seconds, args.eps, args.cpu_seconds, not args.no_cpu)
     if args.warm_train > 0:
         out["warm_start"] = run_warm(args.warm_train, args.warm_test, args.eps)
     print(json.dumps(out), flush=True)

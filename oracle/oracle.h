@@ -83,6 +83,18 @@ void orc_solver_free(orc_solver_t *s);
 void orc_solver_scalings(const orc_solver_t *s, double *con_rescale, double *var_rescale, double *cbr, double *ovr,
                          double *step_size, double *primal_weight);
 
+/* Loop state for the trajectory tests, in the scaled space: the stored PDHG
+ * iterate and the restart anchor (n, m entries; x_p, x_0, the last delta
+ * x written by the fixed-point error or a restart, y_p, y_0; any pointer may
+ * be NULL), and [inner tolerance, fixed-point error, relative primal and
+ * dual residual of the last evaluation, Jacobi tolerance scale]. */
+void orc_solver_vectors(const orc_solver_t *s, double *xp, double *x0, double *dx, double *yp, double *y0);
+void orc_solver_loop_state(const orc_solver_t *s, double *out);
+
+/* The restart rule (reference src/utils.cu:291-322) as a pure function. */
+int orc_restart_decision(const restart_parameters_t *rp, int ef, int total_count, int inner_count, double fpe,
+                         double init_fpe, double last_trial_fpe);
+
 /* ---- cone projections on the scaled space, one block at a time ----
  * x holds the block's slots, d the block's variable scaling (the projection
  * is onto D K; d = 1 gives the Euclidean projection onto K), alpha the power

@@ -775,16 +775,21 @@ static void restart(orc_solver_t *s) {
 }
 
 /* reference src/utils.cu:291-322 */
-static int should_restart(orc_solver_t *s) {
-    const restart_parameters_t *rp = &s->params.restart_params;
-    int ef = s->params.termination_evaluation_frequency;
+int orc_restart_decision(const restart_parameters_t *rp, int ef, int total_count, int inner_count, double fpe,
+                         double init_fpe, double last_trial_fpe) {
     int r = 0;
-    if (s->total_count == ef) r = 1;
-    else if (s->total_count > ef) {
-        if (s->fpe <= rp->sufficient_reduction_for_restart * s->init_fpe) r = 1;
-        if (s->fpe <= rp->necessary_reduction_for_restart * s->init_fpe && s->fpe > s->last_trial_fpe) r = 1;
-        if (s->inner_count >= rp->artificial_restart_threshold * s->total_count) r = 1;
+    if (total_count == ef) r = 1;
+    else if (total_count > ef) {
+        if (fpe <= rp->sufficient_reduction_for_restart * init_fpe) r = 1;
+        if (fpe <= rp->necessary_reduction_for_restart * init_fpe && fpe > last_trial_fpe) r = 1;
+        if (inner_count >= rp->artificial_restart_threshold * total_count) r = 1;
     }
+    return r;
+}
+
+static int should_restart(orc_solver_t *s) {
+    int r = orc_restart_decision(&s->params.restart_params, s->params.termination_evaluation_frequency,
+                                 s->total_count, s->inner_count, s->fpe, s->init_fpe, s->last_trial_fpe);
     s->last_trial_fpe = s->fpe;
     return r;
 }
@@ -1076,6 +1081,22 @@ void orc_solver_scalings(const orc_solver_t *s, double *cr, double *vr, double *
     if (ovr) *ovr = s->w->ovr;
     if (step) *step = s->step_size;
     if (pw) *pw = s->primal_weight;
+}
+
+void orc_solver_vectors(const orc_solver_t *s, double *xp, double *x0, double *dx, double *yp, double *y0) {
+    if (xp) memcpy(xp, s->xp, (size_t)s->n * sizeof(double));
+    if (x0) memcpy(x0, s->x0, (size_t)s->n * sizeof(double));
+    if (dx) memcpy(dx, s->dx, (size_t)s->n * sizeof(double));
+    if (yp) memcpy(yp, s->yp, (size_t)s->m * sizeof(double));
+    if (y0) memcpy(y0, s->y0, (size_t)s->m * sizeof(double));
+}
+
+void orc_solver_loop_state(const orc_solver_t *s, double *out) {
+    out[0] = s->inner_tol;
+    out[1] = s->fpe;
+    out[2] = s->rel_pr;
+    out[3] = s->rel_du;
+    out[4] = s->tol_scale;
 }
 
 /* reference src/pdhg_core_op.cu:1431-1496 */

@@ -223,7 +223,11 @@ __device__ inline void soc_dual_residual_block(const double *pr, int k, const do
 
 // ---------------- exponential cone ----------------
 
+// Multiply-adds are not contracted here: the Newton iteration from rho = 0
+// can settle on different roots when the last bit of a step differs, and the
+// host restatement of this routine is compiled without contraction too.
 __device__ inline void exp_project_point(double r1, double r2, double r3, double d1, double d2, double d3, double *o) {
+#pragma clang fp contract(off)
     const double E = 2.718281828459045;
     const double a1 = r1 / d1, a2 = r2 / d2, a3 = r3 / d3;
     if (a2 > 0.0) {
@@ -770,118 +774,202 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
 }
 
 // ---------------- PSD cone: one workgroup per matrix ----------------
+//
+// Parallel two-sided Jacobi (Golub & Van Loan, sec. 8.5) on the dense
+// symmetric matrix of an svec block.  The order k is padded to an even kp
+// (a zero row and column: a zero eigenvalue that contributes nothing).  A
+// sweep is kp - 1 rounds of the circle (round-robin tournament) ordering; the
+// kp / 2 pairs of a round are disjoint, so their rotations are applied at
+// once.  A round is two phases separated by barriers:
+//   1. each pair (p, q) gets its rotation (c, s) from its 2x2 diagonal block;
+//   2. every 2x2 block A[{p,q},{r,s}] with pair P <= pair R becomes
+//      J_P' B J_R (its mirror is written from the same values, so A stays
+//      exactly symmetric; a diagonal block's off-diagonal entry is set to 0),
+//      and rows p, q of V' become J_P' V'[{p,q}, :].
+// A lives in LDS; V' too when both fit the workgroup's LDS, otherwise in the
+// matrix's slice of the global work buffer (mode 1); with neither fitting
+// (orders above about 128) both use the work buffer (mode 2).  Sweeps run
+// until the off-diagonal mass is at most 1e-30 of the total.  A matrix not
+// converged after kPsdMaxSweeps sweeps (only non-finite input does that) is
+// written as NaN, as the reference writes NaN when its eigensolver fails
+// (reference src/kernels/pdhcg_psd_cone_kernels.cu:263-486).
 
-// Projects the svec block of order k at v + start onto the PSD cone.  The
-// dense matrix and the rotation accumulator live in `work` (2 k^2 doubles per
-// matrix).  Cyclic Jacobi: every (p, q) pair in row order, the rotation
-// applied to rows / columns by the whole workgroup; sweeps until the
-// off-diagonal mass is below 1e-30 of the total (or 100 sweeps).
-template <class X>
-__global__ void __launch_bounds__(kBlock) psd_project_kernel(X *v, const int *starts, int k, double *work,
-                                                              const int *gate) {
+constexpr int kPsdMaxSweeps = 60;
+constexpr int kPsdThreads = 256;
+
+// Workgroup LDS (dynamic) of a projection of order k in a mode.
+__host__ __device__ inline size_t psd_lds_bytes(int k, int mode) {
+    const size_t kp = (size_t)(k + (k & 1)), np = kp / 2;
+    const size_t mats = mode == 0 ? 2 : (mode == 1 ? 1 : 0);
+    return (mats * kp * kp + 2 * np) * sizeof(double) + 2 * np * sizeof(int);
+}
+
+// Global work doubles per matrix in a mode.
+__host__ __device__ inline size_t psd_work_doubles(int k, int mode) {
+    const size_t kp = (size_t)(k + (k & 1));
+    return mode == 0 ? 0 : (mode == 1 ? kp * kp : 2 * kp * kp);
+}
+
+// Row r and column c (r >= c) of svec slot `slot` of an order-k block
+// (lower triangle, column-major).
+__device__ __forceinline__ void psd_slot_rc(int slot, int k, int &r, int &c) {
+    c = 0;
+    int rem = slot;
+    while (rem >= k - c) {
+        rem -= k - c;
+        ++c;
+    }
+    r = c + rem;
+}
+
+template <class X, int Mode>
+__global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const int *starts, int k, double *work,
+                                                                   const int *gate) {
     if (gate && *gate == 0) return;
+    extern __shared__ double psd_lds[];
+    __shared__ double red[2 * (kPsdThreads / kWave)];
+    __shared__ int stop;
     const int mat = blockIdx.x;
     const int start = starts[mat];
-    double *A = work + (size_t)mat * 2 * k * k;
-    double *V = A + (size_t)k * k;
-    __shared__ double cs[2];
-    __shared__ double red[kBlock / kWave];
-    __shared__ int stop;
-    const int L = k * (k + 1) / 2;
+    const int tid = threadIdx.x, nt = blockDim.x;
     if (k == 1) {
-        if (threadIdx.x == 0) v[start] = (X)fmax((double)v[start], 0.0);
+        if (tid == 0) v[start] = (X)fmax((double)v[start], 0.0);
         return;
     }
-    // svec -> dense (lower triangle, column-major, off-diagonal / sqrt 2)
-    for (int slot = threadIdx.x; slot < L; slot += blockDim.x) {
-        int c = 0, rem = slot;
-        while (rem >= k - c) {
-            rem -= k - c;
-            ++c;
-        }
-        const int r = c + rem;
+    const int kp = k + (k & 1), np = kp / 2, m = kp - 1;
+    const size_t kk = (size_t)kp * kp;
+    double *wbase = work + (size_t)mat * psd_work_doubles(k, Mode);
+    double *A = Mode == 2 ? wbase : psd_lds;
+    double *Vt = Mode == 0 ? psd_lds + kk : (Mode == 1 ? wbase : wbase + kk);
+    double *cs = psd_lds + (Mode == 0 ? 2 * kk : (Mode == 1 ? kk : 0));
+    int *pq = reinterpret_cast<int *>(cs + 2 * np);
+    const int L = k * (k + 1) / 2;
+
+    // svec -> dense (off-diagonal entries / sqrt 2), zero padding, V' = I
+    for (size_t i = tid; i < kk; i += nt) {
+        A[i] = 0.0;
+        Vt[i] = (i / kp == i % kp) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int slot = tid; slot < L; slot += nt) {
+        int r, c;
+        psd_slot_rc(slot, k, r, c);
         double val = (double)v[start + slot];
         if (r != c) val *= kInvSqrt2;
-        A[r * k + c] = val;
-        A[c * k + r] = val;
+        A[(size_t)r * kp + c] = val;
+        A[(size_t)c * kp + r] = val;
     }
-    for (int i = threadIdx.x; i < k * k; i += blockDim.x) V[i] = (i / k == i % k) ? 1.0 : 0.0;
     __syncthreads();
-    for (int sweep = 0; sweep < 100; ++sweep) {
+
+    bool converged = false;
+    for (int sweep = 0; sweep <= kPsdMaxSweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
-        for (int i = threadIdx.x; i < k * k; i += blockDim.x) {
+        for (size_t i = tid; i < kk; i += nt) {
             const double a = A[i];
             tot += a * a;
-            if (i / k != i % k) off += a * a;
+            if (i / kp != i % kp) off += a * a;
         }
         off = wave_sum(off);
         tot = wave_sum(tot);
-        if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = off;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double o = 0.0;
-            for (int w = 0; w < kBlock / kWave; ++w) o += red[w];
-            red[0] = o;
+        if (tid % kWave == 0) {
+            red[2 * (tid / kWave)] = off;
+            red[2 * (tid / kWave) + 1] = tot;
         }
         __syncthreads();
-        const double off_total = red[0];
-        __syncthreads();
-        if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = tot;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
-            stop = (off_total <= 1e-30 * t || off_total == 0.0) ? 1 : 0;
-        }
-        __syncthreads();
-        if (stop) break;
-        for (int p = 0; p < k - 1; ++p)
-            for (int q = p + 1; q < k; ++q) {
-                if (threadIdx.x == 0) {
-                    const double apq = A[p * k + q];
-                    if (apq == 0.0) {
-                        cs[0] = 1.0;
-                        cs[1] = 0.0;
-                    } else {
-                        const double theta = (A[q * k + q] - A[p * k + p]) / (2.0 * apq);
-                        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        const double c = 1.0 / sqrt(t * t + 1.0);
-                        cs[0] = c;
-                        cs[1] = t * c;
-                    }
-                }
-                __syncthreads();
-                const double c = cs[0], s = cs[1];
-                if (s != 0.0) {
-                    for (int r = threadIdx.x; r < k; r += blockDim.x) {
-                        const double arp = A[r * k + p], arq = A[r * k + q];
-                        A[r * k + p] = c * arp - s * arq;
-                        A[r * k + q] = s * arp + c * arq;
-                        const double vrp = V[r * k + p], vrq = V[r * k + q];
-                        V[r * k + p] = c * vrp - s * vrq;
-                        V[r * k + q] = s * vrp + c * vrq;
-                    }
-                    __syncthreads();
-                    for (int r = threadIdx.x; r < k; r += blockDim.x) {
-                        const double apr = A[p * k + r], aqr = A[q * k + r];
-                        A[p * k + r] = c * apr - s * aqr;
-                        A[q * k + r] = s * apr + c * aqr;
-                    }
-                }
-                __syncthreads();
+        if (tid == 0) {
+            double o = 0.0, t = 0.0;
+            for (int w = 0; w < nt / kWave; ++w) {
+                o += red[2 * w];
+                t += red[2 * w + 1];
             }
+            stop = (o <= 1e-30 * t || o == 0.0) ? 1 : 0;
+        }
+        __syncthreads();
+        if (stop) {
+            converged = true;
+            break;
+        }
+        if (sweep == kPsdMaxSweeps) break;
+        for (int t = 0; t < m; ++t) {
+            // phase 1: the round's pairs and rotations
+            for (int i = tid; i < np; i += nt) {
+                int a, b;
+                if (i == 0) {
+                    a = t;
+                    b = m;
+                } else {
+                    a = (t + i) % m;
+                    b = (t - i + m) % m;
+                }
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                const double apq = A[(size_t)p * kp + q];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[(size_t)q * kp + q] - A[(size_t)p * kp + p]) / (2.0 * apq);
+                    const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    s = tt * c;
+                }
+                cs[2 * i] = c;
+                cs[2 * i + 1] = s;
+                pq[2 * i] = p;
+                pq[2 * i + 1] = q;
+            }
+            __syncthreads();
+            // phase 2: J_P' B J_R on the blocks with P <= R, and V'
+            for (int w = tid; w < np * np; w += nt) {
+                const int P = w / np, R = w - P * np;
+                if (P > R) continue;
+                const int p = pq[2 * P], q = pq[2 * P + 1], r = pq[2 * R], s = pq[2 * R + 1];
+                const double c1 = cs[2 * P], s1 = cs[2 * P + 1], c2 = cs[2 * R], s2 = cs[2 * R + 1];
+                double *Ap = A + (size_t)p * kp, *Aq = A + (size_t)q * kp;
+                const double b00 = Ap[r], b01 = Ap[s], b10 = Aq[r], b11 = Aq[s];
+                const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
+                const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
+                const double n00 = c2 * m00 - s2 * m01, n01 = s2 * m00 + c2 * m01;
+                const double n10 = c2 * m10 - s2 * m11, n11 = s2 * m10 + c2 * m11;
+                if (P == R) {
+                    Ap[p] = n00;
+                    Aq[q] = n11;
+                    Ap[q] = 0.0;
+                    Aq[p] = 0.0;
+                } else {
+                    Ap[r] = n00;
+                    Ap[s] = n01;
+                    Aq[r] = n10;
+                    Aq[s] = n11;
+                    A[(size_t)r * kp + p] = n00;
+                    A[(size_t)s * kp + p] = n01;
+                    A[(size_t)r * kp + q] = n10;
+                    A[(size_t)s * kp + q] = n11;
+                }
+            }
+            for (int w = tid; w < np * kp; w += nt) {
+                const int P = w / kp, i = w - P * kp;
+                const double c = cs[2 * P], s = cs[2 * P + 1];
+                if (s == 0.0) continue;
+                double *vp = Vt + (size_t)pq[2 * P] * kp, *vq = Vt + (size_t)pq[2 * P + 1] * kp;
+                const double a = vp[i], b = vq[i];
+                vp[i] = c * a - s * b;
+                vq[i] = s * a + c * b;
+            }
+            __syncthreads();
+        }
     }
     // svec of V max(diag(A), 0) V'
-    for (int slot = threadIdx.x; slot < L; slot += blockDim.x) {
-        int c = 0, rem = slot;
-        while (rem >= k - c) {
-            rem -= k - c;
-            ++c;
-        }
-        const int r = c + rem;
+    for (int slot = tid; slot < L; slot += nt) {
+        int r, c;
+        psd_slot_rc(slot, k, r, c);
         double val = 0.0;
-        for (int e = 0; e < k; ++e) val += fmax(A[e * k + e], 0.0) * V[r * k + e] * V[c * k + e];
-        if (r != c) val *= kSqrt2;
+        if (converged) {
+            for (int e = 0; e < kp; ++e) {
+                const double lam = A[(size_t)e * kp + e];
+                if (lam > 0.0) val += lam * Vt[(size_t)e * kp + r] * Vt[(size_t)e * kp + c];
+            }
+            if (r != c) val *= kSqrt2;
+        } else {
+            val = NAN;
+        }
         v[start + slot] = (X)val;
     }
 }

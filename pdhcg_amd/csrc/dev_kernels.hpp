@@ -119,6 +119,18 @@ __global__ void __launch_bounds__(kBlock) halpern_kernel(const X *x0, X *x, cons
     }
 }
 
+// out = (double)in on two vectors at once (the FP32 iterates handed to the
+// FP64 evaluation of the mixed-precision solver).
+template <class X>
+__global__ void __launch_bounds__(kBlock) widen2_kernel(const X *a, double *a_out, int na, const X *b, double *b_out,
+                                                         int nb) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (long long)na + nb;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (i < na) a_out[i] = (double)a[i];
+        else b_out[i - na] = (double)b[i - na];
+    }
+}
+
 template <class X> __global__ void __launch_bounds__(kBlock) clamp_kernel(X *x, const X *lb, const X *ub, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         x[i] = (X)fmax((double)lb[i], fmin((double)x[i], (double)ub[i]));

@@ -48,4 +48,11 @@ void fill_default_parameters(pdhg_parameters_t *p);
 // Fixed-section validation (reference src/pdhcg.c:729-892).
 int validate_fixed_sections(const qp_problem_t *problem);
 
+// Diagonal quadratic constraints lifted into SOC / RSOC blocks, and the
+// result of the lifted problem cut back to the original dimensions
+// (reference src/qcqp_transform.c; host_qcqp.cpp).  NULL with a message when
+// the problem cannot be lifted.
+qp_problem_t *lift_quadratic_constraints(const qp_problem_t *problem, cone_type_t form);
+void restore_result_dimensions(pdhcg_result_t *result, const qp_problem_t *original);
+
 }  // namespace pdhcg

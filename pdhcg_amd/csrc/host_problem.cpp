@@ -645,13 +645,6 @@ int set_cone_fixed(qp_problem_t *p, int cone_idx, int slot, double value) {
     return 0;
 }
 
-qp_problem_t *qcqp_to_socp_qp(const qp_problem_t *, cone_type_t) {
-    std::fprintf(stderr,
-                 "[qcqp_to_socp_qp] quadratic constraints are not supported by this library; "
-                 "reformulate them as second-order cones before create_qp_problem.\n");
-    return nullptr;
-}
-
 pdhcg_result_t *solve_qp_problem_distributed(const pdhg_parameters_t *, const qp_problem_t *) {
     std::fprintf(stderr,
                  "Error: distributed solve is not available in this build (single-GPU library); "

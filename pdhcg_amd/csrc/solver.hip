@@ -105,9 +105,38 @@ struct RunStats {
 // internal/internal_types.h:106-127, dispatch src/cone_dispatch.cu).
 struct PsdGroup {
     int order = 0, count = 0;
+    int mode = 0;            // where the Jacobi keeps A and V' (psd_project_kernel)
+    size_t lds = 0;          // dynamic LDS bytes per workgroup
     int *starts = nullptr;
-    double *work = nullptr;  // 2 order^2 doubles per matrix
+    double *work = nullptr;  // psd_work_doubles(order, mode) per matrix (null in mode 0)
 };
+
+// LDS a workgroup may allocate on the current device, less the PSD kernel's
+// static arrays and a margin.
+inline size_t psd_lds_budget() {
+    int dev = 0, lds = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return lds > 1024 ? (size_t)lds - 1024 : 0;
+}
+
+// Launches the PSD projection of one order group (the kernel instance that
+// matches the group's mode; dynamic LDS above the default limit is requested
+// once per instance).
+template <class X> void psd_launch(const PsdGroup &g, X *v, const int *gate, hipStream_t st) {
+    static bool raised[3] = {false, false, false};
+    auto go = [&](auto kernel, int mode) {
+        if (!raised[mode] && g.lds > 0) {
+            HIP_OK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)psd_lds_budget()));
+            raised[mode] = true;
+        }
+        hipLaunchKernelGGL(kernel, dim3(g.count), dim3(kPsdThreads), g.lds, st, v, g.starts, g.order, g.work, gate);
+    };
+    if (g.mode == 0) go(psd_project_kernel<X, 0>, 0);
+    else if (g.mode == 1) go(psd_project_kernel<X, 1>, 1);
+    else go(psd_project_kernel<X, 2>, 2);
+}
 
 class DevConeSet {
   public:
@@ -234,7 +263,14 @@ class DevConeSet {
             g.order = ps[i].first;
             g.count = (int)starts.size();
             g.starts = dupload<int>(starts, starts.size());
-            g.work = dalloc<double>((size_t)2 * g.order * g.order * g.count);
+            const size_t budget = psd_lds_budget();
+            g.mode = psd_lds_bytes(g.order, 0) <= budget ? 0 : (psd_lds_bytes(g.order, 1) <= budget ? 1 : 2);
+            // PDHCG_PSD_MODE=1|2 moves V' (1) or A and V' (2) to the work buffer
+            // on orders that would fit LDS (the tests run every mode)
+            if (const char *fm = std::getenv("PDHCG_PSD_MODE")) g.mode = std::max(g.mode, std::min(2, std::atoi(fm)));
+            g.lds = psd_lds_bytes(g.order, g.mode);
+            const size_t wd = psd_work_doubles(g.order, g.mode) * (size_t)g.count;
+            if (wd) g.work = dalloc<double>(wd);
             groups.push_back(g);
             i = j;
         }
@@ -274,9 +310,7 @@ class DevConeSet {
         if (lsec.count)
             hipLaunchKernelGGL(section_long_project_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, v, d,
                                (const X *)nullptr, 0.0, fixed, lsec, lsec_scratch, lsec_off, gate);
-        for (const PsdGroup &g : groups)
-            hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, v, g.starts, g.order, g.work,
-                               gate);
+        for (const PsdGroup &g : groups) psd_launch<X>(g, v, gate, st);
     }
 
     // Cone step of the diagonal-Q prox: projection onto diag(d) K in the
@@ -342,9 +376,7 @@ class DevConeSet {
         if (psd) {
             hipLaunchKernelGGL(psd_rc_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, scratch, dres, c, qx, aty,
                                mask, n);
-            for (const PsdGroup &g : groups)
-                hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kBlock), 0, st, scratch, g.starts,
-                                   g.order, g.work, nullptr);
+            for (const PsdGroup &g : groups) psd_launch<X>(g, scratch, nullptr, st);
             hipLaunchKernelGGL(psd_dres_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, dres, scratch, d, mask,
                                n);
         }
@@ -398,12 +430,19 @@ class SolverBase {
 };
 
 template <class V, class X> class Solver final : public SolverBase {
+    template <class, class> friend class Solver;
+
   public:
-    Solver(const ScaledProblem &sp, const pdhg_parameters_t &params);
+    // eval_only: an FP64 evaluation copy of the problem for the mixed-precision
+    // solver (KKT residuals, infeasibility rays, result); it shares the
+    // owner's stream and runs no iterations of its own.
+    Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, hipStream_t shared = nullptr,
+           bool eval_only = false);
     ~Solver() override;
     int run(int iterations) override;
     pdhcg_result_t *result(const qp_problem_t *original) override;
     void counts(int *total, int *inner) override {
+        if (promoted_) return eval_->counts(total, inner);
         read_scalars();
         *total = total_count_;
         *inner = host_inner_ + hs_->inner_total;
@@ -464,6 +503,23 @@ template <class V, class X> class Solver final : public SolverBase {
 
     pdhg_parameters_t P_;
     hipStream_t st_ = nullptr;
+    bool own_stream_ = true, eval_only_ = false;
+    // Mixed precision: the termination test, the infeasibility rays and the
+    // returned point are evaluated in FP64 on FP64 problem data, from the
+    // FP32 iterates widened exactly (null in the FP64 solver).
+    Solver<double, double> *eval_ = nullptr;
+    void eval_residual();
+    void eval_infeasibility();
+    // FP32 iterates stop improving at a floor set by their rounding (a
+    // relative KKT error near 1e-6 on problems whose |A||x| is large next to
+    // the right-hand sides).  When the FP64-evaluated error has not halved
+    // for stall_iters_ iterations, the state moves to the evaluation copy,
+    // which finishes the solve with FP64 iterates (promoted_).
+    bool promoted_ = false;
+    double best_err_ = INFINITY;
+    int best_err_iter_ = 0, stall_iters_ = 1000;
+    bool stalled(int tc);
+    void promote();
     int n_ = 0, m_ = 0, k_ = 0;
     quad_obj_type_t qtype_ = PDHCG_NON_Q;
     bool is_lp_ = true;
@@ -567,9 +623,15 @@ template <class V, class X> class Solver final : public SolverBase {
     KernelTimer timer_;
 };
 
-template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params) : P_(params) {
+template <class V, class X>
+Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, hipStream_t shared, bool eval_only)
+    : P_(params) {
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    eval_only_ = eval_only;
+    if (shared) {
+        st_ = shared;
+        own_stream_ = false;
+    } else HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     n_ = sp.n;
     m_ = sp.m;
     k_ = sp.k;
@@ -588,6 +650,17 @@ template <class V, class X> Solver<V, X>::Solver(const ScaledProblem &sp, const 
     HIP_OK(hipHostMalloc((void **)&bb_poll_, 4 * sizeof(int), hipHostMallocDefault));
     for (hipEvent_t &e : bb_ev_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     upload(sp);
+    if (eval_only_) {
+        HIP_OK(hipStreamSynchronize(st_));
+        return;
+    }
+    if constexpr (std::is_same<X, float>::value) {
+        eval_ = new Solver<double, double>(sp, params, st_, true);
+        // PDHCG_MIXED_STALL=N: iterations without a halving of the FP64 KKT
+        // error before the FP64 finish (0: never)
+        const char *e = std::getenv("PDHCG_MIXED_STALL");
+        stall_iters_ = e ? std::atoi(e) : std::max(1000, 5 * P_.termination_evaluation_frequency);
+    }
 
     // quadratic term information (reference src/solver_state.cu:380-420)
     if (qtype_ == PDHCG_SPARSE_Q) {
@@ -628,7 +701,7 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     const int panel_mode = pe ? std::atoi(pe) : -1;
     const size_t kPanelMinBytes = (size_t)4 << 20;
     auto want_panels = [&](int cols, int nnz, int rows) {
-        if (panel_mode == 0) return false;
+        if (panel_mode == 0 || eval_only_) return false;
         if (panel_mode == 1) return rows > 0;
         return (size_t)cols * sizeof(X) > kPanelMinBytes && nnz >= 4 * rows;
     };
@@ -658,9 +731,10 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     }
     if (qtype_ == PDHCG_DIAG_Q) qdiag_ = dupload<X>(sp.qdiag, (size_t)n_);
     if (qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
-        if (k_ > kRxMaxRank) fail("low-rank term of rank %d exceeds the supported maximum %d", k_, kRxMaxRank);
+        // the dense tile pass keeps one LDS slot per row of R, so ranks above
+        // kRxMaxRank take the CSR products, which have no rank limit
         const long long dense_elems = (long long)k_ * n_;
-        r_dense_ = dense_elems <= 4LL * sp.R.nnz() + 4096;
+        r_dense_ = dense_elems <= 4LL * sp.R.nnz() + 4096 && k_ <= kRxMaxRank;
         if (r_dense_) {
             rx_chunks_ = rx_chunks(n_);
             std::vector<double> Rd((size_t)rx_chunks_ * kRxWaveCols * k_, 0.0);
@@ -738,7 +812,7 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     dx_ = dzeros<X>(n_);
     dy_ = dzeros<X>(m_);
     qx_ = dzeros<X>(n_);
-    tune_spmv();
+    if (!eval_only_) tune_spmv();
 }
 
 // Mean duration (ms) of a launch sequence on the solver stream: one
@@ -881,6 +955,7 @@ template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &s
 
 template <class V, class X> Solver<V, X>::~Solver() {
     if (st_) (void)hipStreamSynchronize(st_);
+    delete eval_;
     for (GraphWindow &g : graphs_) release_graph(g);
     if (dyn_) (void)hipFree(dyn_);
     free_binned(A_);
@@ -904,7 +979,7 @@ template <class V, class X> Solver<V, X>::~Solver() {
     if (bb_poll_) (void)hipHostFree(bb_poll_);
     for (hipEvent_t e : bb_ev_)
         if (e) (void)hipEventDestroy(e);
-    if (st_) (void)hipStreamDestroy(st_);
+    if (st_ && own_stream_) (void)hipStreamDestroy(st_);
 }
 
 // ||A||_2 by power iteration on A A' (reference src/pdhg_core_op.cu:1673-1788).
@@ -1369,7 +1444,118 @@ template <class V, class X> bool Solver<V, X>::should_restart() {
 }
 
 // reference src/pdhg_core_op.cu:1037-1294
+// FP64 evaluation of the FP32 iterates (mixed precision): x_bar, y_bar and,
+// for an LP, the dual slack of the last primal step are widened exactly into
+// the evaluation copy, whose FP64 products and reductions give the residuals.
+template <class V, class X> void Solver<V, X>::eval_residual() {
+    Solver<double, double> &E = *eval_;
+    hipLaunchKernelGGL(widen2_kernel<X>, dim3(full_grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, E.xp_, n_,
+                       yp_, E.yp_, m_);
+    if (is_lp_)
+        hipLaunchKernelGGL(widen2_kernel<X>, dim3(full_grid(n_)), dim3(kBlock), 0, st_, dslack_, E.dslack_, n_,
+                           (const X *)nullptr, (double *)nullptr, 0);
+    HIP_OK(hipGetLastError());
+    E.step_ = step_;
+    E.pw_ = pw_;
+    E.residual();
+    abs_pr_ = E.abs_pr_;
+    rel_pr_ = E.rel_pr_;
+    abs_du_ = E.abs_du_;
+    rel_du_ = E.rel_du_;
+    pobj_ = E.pobj_;
+    dobj_ = E.dobj_;
+    gap_ = E.gap_;
+    rel_gap_ = E.rel_gap_;
+}
+
+template <class V, class X> void Solver<V, X>::eval_infeasibility() {
+    Solver<double, double> &E = *eval_;
+    hipLaunchKernelGGL(widen2_kernel<X>, dim3(full_grid((long long)n_ + m_)), dim3(kBlock), 0, st_, dx_, E.dx_, n_,
+                       dy_, E.dy_, m_);
+    HIP_OK(hipGetLastError());
+    E.infeasibility();
+    pray_lin_obj_ = E.pray_lin_obj_;
+    dray_obj_ = E.dray_obj_;
+    max_pray_ = E.max_pray_;
+    max_dray_ = E.max_dray_;
+}
+
+template <class V, class X> bool Solver<V, X>::stalled(int tc) {
+    if (stall_iters_ <= 0) return false;
+    const double err = std::fmax(rel_pr_, std::fmax(rel_du_, rel_gap_));
+    if (err < 0.5 * best_err_) {
+        best_err_ = err;
+        best_err_iter_ = tc;
+        return false;
+    }
+    return tc - best_err_iter_ >= stall_iters_;
+}
+
+// Hands the loop state to the FP64 evaluation copy: the iterates widened
+// exactly, the inner-solve buffers, and every scalar of the outer loop, so
+// that it continues the same iteration in FP64.
+template <class V, class X> void Solver<V, X>::promote() {
+    Solver<double, double> &E = *eval_;
+    auto widen = [&](const X *a, double *ao, int na, const X *b, double *bo, int nb) {
+        hipLaunchKernelGGL(widen2_kernel<X>, dim3(full_grid((long long)na + nb)), dim3(kBlock), 0, st_, a, ao, na, b,
+                           bo, nb);
+    };
+    widen(x_, E.x_, n_, y_, E.y_, m_);
+    widen(xp_, E.xp_, n_, yp_, E.yp_, m_);
+    widen(xr_, E.xr_, n_, yr_, E.yr_, m_);
+    widen(x0_, E.x0_, n_, y0_, E.y0_, m_);
+    widen(dx_, E.dx_, n_, dy_, E.dy_, m_);
+    widen(dslack_, E.dslack_, n_, (const X *)nullptr, (double *)nullptr, 0);
+    HIP_OK(hipGetLastError());
+    if (bb_) {
+        E.bb_ = true;
+        E.precond_ = precond_;
+        E.grad_ = dzeros<double>(n_);
+        E.dirn_ = dzeros<double>(n_);
+        if (has_cones_) E.snap_ = dzeros<double>(n_);
+        if (precond_) {
+            E.diag_h_ = dalloc<double>(n_);
+            HIP_OK(hipMemcpyAsync(E.diag_h_, diag_h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+            E.mdiag_ = dzeros<double>(n_);
+            E.minv_ = dzeros<double>(n_);
+        }
+    }
+    E.inner_tol_ = inner_tol_;
+    E.inner_min_tol_ = inner_min_tol_;
+    E.inner_limit_ = inner_limit_;
+    E.cached_inv_tau_ = -1.0;
+    E.bb_pred_ = bb_pred_;
+    E.q_norm_ = q_norm_;
+    E.nonconvexity_ = nonconvexity_;
+    E.step_ = step_;
+    E.pw_ = pw_;
+    E.pw_err_sum_ = pw_err_sum_;
+    E.pw_last_err_ = pw_last_err_;
+    E.best_pw_ = best_pw_;
+    E.best_gap_ = best_gap_;
+    E.fpe_ = fpe_;
+    E.init_fpe_ = init_fpe_;
+    E.last_trial_fpe_ = last_trial_fpe_;
+    E.total_count_ = total_count_;
+    E.inner_count_ = inner_count_;
+    E.is_major_ = is_major_;
+    E.do_restart_ = do_restart_;
+    E.max_pray_ = max_pray_;
+    E.max_dray_ = max_dray_;
+    E.pray_lin_obj_ = pray_lin_obj_;
+    E.dray_obj_ = dray_obj_;
+    E.reason_ = reason_;
+    E.cum_time_ = cum_time_;
+    E.t_start_ = t_start_;
+    read_scalars();
+    E.host_inner_ = host_inner_ + hs_->inner_total;
+    E.graph_on_ = false;
+    HIP_OK(hipStreamSynchronize(st_));
+    promoted_ = true;
+}
+
 template <class V, class X> void Solver<V, X>::residual() {
+    if (eval_) return eval_residual();
     const bool linf = P_.optimality_norm == NORM_TYPE_L_INF;
     const double tau = step_ / pw_;
     apply_A(xp_, EpiStore<X>{ax_}, st_);
@@ -1459,6 +1645,7 @@ template <class V, class X> void Solver<V, X>::residual() {
 
 // reference src/pdhg_core_op.cu:1296-1429
 template <class V, class X> void Solver<V, X>::infeasibility() {
+    if (eval_) return eval_infeasibility();
     hipLaunchKernelGGL(ray_project_kernel<X>, dim3(grid((long long)n_ + m_)), dim3(kBlock), 0, st_, dx_, lb_, ub_, dy_,
                        cl_, cu_, n_, m_, red());
     fold<0, 2>(grid((long long)n_ + m_));
@@ -1536,6 +1723,7 @@ template <class V, class X> void Solver<V, X>::check_termination() {
 
 // Main loop body, reference src/solver.cu:121-169.
 template <class V, class X> int Solver<V, X>::run(int iterations) {
+    if (promoted_) return eval_->run(iterations);
     const int ef = P_.termination_evaluation_frequency;
     for (int done = 0; done < iterations; ++done) {
         if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
@@ -1565,6 +1753,13 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
                             host_inner_ + hs_->inner_total, cum_time_, pobj_, dobj_, abs_pr_, abs_du_, gap_, rel_pr_,
                             rel_du_, rel_gap_);
             if (reason_ != TERMINATION_REASON_UNSPECIFIED) break;
+            if (eval_ && stalled(tc)) {
+                promote();
+                if (P_.verbose >= 1)
+                    std::printf("[mixed] FP32 iterates stalled at a KKT error of %.1e; FP64 iterates from iteration %d\n",
+                                best_err_, tc);
+                return eval_->run(iterations - done);
+            }
         }
         if (is_major_ || tc == 0) {
             do_restart_ = should_restart();
@@ -1594,6 +1789,23 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
 
 // reference src/pdhg_core_op.cu:1431-1496
 template <class V, class X> pdhcg_result_t *Solver<V, X>::result(const qp_problem_t *orig) {
+    if (promoted_) return eval_->result(orig);
+    if (eval_) {
+        // the returned point, reduced costs and residuals come from the FP64
+        // evaluation copy; counters, times and status are this solver's
+        Solver<double, double> &E = *eval_;
+        hipLaunchKernelGGL(widen2_kernel<X>, dim3(full_grid((long long)n_ + m_)), dim3(kBlock), 0, st_, xp_, E.xp_,
+                           n_, yp_, E.yp_, m_);
+        HIP_OK(hipGetLastError());
+        pdhcg_result_t *r = E.result(orig);
+        read_scalars();
+        r->total_count = total_count_;
+        r->total_inner_count = host_inner_ + hs_->inner_total;
+        r->rescaling_time_sec = rescale_time_;
+        r->cumulative_time_sec = cum_time_;
+        r->termination_reason = reason_;
+        return r;
+    }
     apply_At(yp_, EpiStore<X>{aty_}, st_);
     q_product(xp_, qx_, nullptr);
     double *xo = dalloc<double>(n_), *yo = dalloc<double>(m_), *rco = dalloc<double>(n_);
@@ -1678,8 +1890,14 @@ void final_log(const pdhcg_result_t *r, const pdhg_parameters_t &p) {  // refere
 
 struct pdhcg_solver {
     std::unique_ptr<SolverBase> impl;
-    const qp_problem_t *problem;
+    const qp_problem_t *problem;   // what the device solves (the lifted copy for a QCQP)
+    const qp_problem_t *original;  // the caller's problem
+    qp_problem_t *lifted = nullptr;
     pdhg_parameters_t params;
+    ~pdhcg_solver() {
+        impl.reset();
+        if (lifted) qp_problem_free(lifted);
+    }
 };
 
 extern "C" {
@@ -1697,21 +1915,37 @@ pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_paramet
         std::fprintf(stderr, "[interface] pdhcg_solver_create: invalid arguments.\n");
         return nullptr;
     }
-    if (prob->num_quadratic_constraints > 0) {
-        std::fprintf(stderr, "Error: QCQP -> SOCP transformation is not available; cannot solve.\n");
-        return nullptr;
-    }
+    pdhcg_solver *s = nullptr;
     try {
-        auto *s = new pdhcg_solver();
-        s->problem = prob;
+        s = new pdhcg_solver();
+        s->problem = s->original = prob;
         if (params) s->params = *params;
         else fill_default_parameters(&s->params);
-        ScaledProblem sp = scale_problem(s->params, *prob);
+        // quadratic constraints are lifted into cone blocks first
+        // (reference src/solver.cu:44-66)
+        if (prob->num_quadratic_constraints > 0) {
+            s->lifted = lift_quadratic_constraints(prob, s->params.default_cone_type);
+            if (!s->lifted) {
+                std::fprintf(stderr, "Error: QCQP -> SOCP transformation failed; cannot solve.\n");
+                delete s;
+                return nullptr;
+            }
+            if (s->params.verbose >= 1)
+                std::fprintf(stderr, "[QCQP] %d quadratic constraint(s) reformulated as %d %s SOC block(s); extended "
+                             "problem: %d vars, %d rows, %d nnz.\n", prob->num_quadratic_constraints,
+                             s->lifted->cones.num_cones,
+                             s->params.default_cone_type == CONE_STANDARD_SOC ? "standard" : "rotated",
+                             s->lifted->num_variables, s->lifted->num_constraints,
+                             s->lifted->constraint_matrix_num_nonzeros);
+            s->problem = s->lifted;
+        }
+        ScaledProblem sp = scale_problem(s->params, *s->problem);
         if (precision == PDHCG_PRECISION_MIXED) s->impl.reset(new Solver<float, float>(sp, s->params));
         else s->impl.reset(new Solver<double, double>(sp, s->params));
         return s;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[interface] solver setup failed: %s\n", e.what());
+        delete s;
         return nullptr;
     }
 }
@@ -1751,7 +1985,9 @@ int pdhcg_solver_synchronize(pdhcg_solver_t *s) {
 pdhcg_result_t *pdhcg_solver_result(pdhcg_solver_t *s) {
     if (!s) return nullptr;
     try {
-        return s->impl->result(s->problem);
+        pdhcg_result_t *r = s->impl->result(s->problem);
+        if (s->lifted) restore_result_dimensions(r, s->original);  // reference src/solver.cu:199
+        return r;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[interface] result extraction failed: %s\n", e.what());
         return nullptr;

@@ -36,6 +36,10 @@ class QPData:
     primal_start: Optional[np.ndarray] = None
     dual_start: Optional[np.ndarray] = None
     cone_fixed: Sequence[tuple] = ()  # (cone index, slot, value) applied with set_cone_fixed
+    # (row, Q) pairs: row `row` of A also carries x'Qx (Q n x n, diagonal for
+    # the SOC lifting); attached to the assembled problem's quadratic-constraint
+    # fields, which create_qp_problem has no arguments for
+    quadratic_constraints: Sequence[tuple] = ()
 
     @property
     def n(self) -> int:
@@ -101,6 +105,8 @@ def build(lib, prefix: str, p: QPData) -> Optional[Handle]:
     if not ptr:
         return None
     h = Handle(lib, prefix, ptr, [])
+    if p.quadratic_constraints:
+        attach_quadratic_constraints(ptr, p.quadratic_constraints, p.n)
     for cone, slot, value in p.cone_fixed:
         if getattr(lib, prefix + "set_cone_fixed")(ptr, int(cone), int(slot), float(value)) != 0:
             h.close()
@@ -109,6 +115,46 @@ def build(lib, prefix: str, p: QPData) -> Optional[Handle]:
         ps, ds = arr(p.primal_start), arr(p.dual_start)
         getattr(lib, prefix + "set_start_values")(ptr, capi.as_ptr(ps), capi.as_ptr(ds))
     return h
+
+
+_libc = C.CDLL(None)
+_libc.malloc.restype = C.c_void_p
+_libc.malloc.argtypes = [C.c_size_t]
+
+
+def _c_copy(arr, ctype):
+    """A malloc'd copy of arr (the problem's free() releases it)."""
+    a = np.ascontiguousarray(arr, dtype=np.int32 if ctype is C.c_int else np.float64)
+    ptr = _libc.malloc(max(a.nbytes, 1))
+    if not ptr:
+        raise MemoryError("malloc failed")
+    C.memmove(ptr, a.ctypes.data, a.nbytes)
+    return C.cast(ptr, C.POINTER(ctype))
+
+
+def attach_quadratic_constraints(ptr, qcs, n: int) -> None:
+    """Sets the quadratic-constraint fields of an assembled qp_problem_t
+    (reference include/pdhcg_types.h:123-126): for each (row, Q), the row index
+    and Q as an n x n CSR in malloc'd arrays owned by the problem."""
+    import scipy.sparse as sp
+    K = len(qcs)
+    prob = ptr.contents
+    rows = _c_copy([int(r) for r, _ in qcs], C.c_int)
+    mats = C.cast(_libc.malloc(K * C.sizeof(C.c_void_p)), C.POINTER(C.POINTER(capi.CsrComponent)))
+    nnzs = []
+    for i, (_, Q) in enumerate(qcs):
+        Qc = sp.csr_matrix(Q, shape=(n, n))
+        Qc.sort_indices()
+        comp = C.cast(_libc.malloc(C.sizeof(capi.CsrComponent)), C.POINTER(capi.CsrComponent))
+        comp.contents.row_ptr = _c_copy(Qc.indptr, C.c_int)
+        comp.contents.col_ind = _c_copy(Qc.indices, C.c_int)
+        comp.contents.val = _c_copy(Qc.data, C.c_double)
+        mats[i] = comp
+        nnzs.append(Qc.nnz)
+    prob.num_quadratic_constraints = K
+    prob.quadratic_constraint_row_indices = rows
+    prob.quadratic_constraint_matrices = mats
+    prob.quadratic_constraint_matrix_num_nonzeros = _c_copy(nnzs, C.c_int)
 
 
 def default_params(lib, prefix: str = "", **overrides) -> capi.pdhg_parameters_t:

@@ -12,6 +12,7 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
+from . import capi
 from .problem import QPData
 
 
@@ -229,3 +230,104 @@ class ParametricFamily:
             Z[i] = rng.standard_normal(self.z_dim)
             problems.append(self.instance(Z[i], (seed, i, 1)))
         return Z, problems
+
+
+def conic_mix(scale: float = 1.0, seed: int = 1002, psd_orders=(16, 128), with_affine: bool = True) -> QPData:
+    """Conic mix: SOC, RSOC, exponential, power and PSD blocks plus an affine
+    SOC block F x + g, with a sparse positive definite Q.
+
+    The stated parts of this family (30% of n in SOC blocks of U{3..64} slots,
+    10% in RSOC blocks of U{3..32} slots, 100k exponential cones, 50k power
+    cones with alpha ~ U(0.1, 0.9), 200 PSD blocks of order U{16..128}) add up
+    to about 1.8M variables, not 600k.  Resolution: the exponential, power
+    and PSD counts are scaled by one factor (1/3) so that n is about 600k at
+    scale 1: 33,333 exponential cones, 16,667 power cones and 67 PSD blocks
+    (about 212k svec slots); SOC and RSOC blocks then fill 30% and 10% of n.
+    `scale` multiplies every count.  Block order is shuffled.
+
+    A: m = n / 4 rows with 2M nnz (scale 1) at uniform positions, N(0, 1);
+    row bounds A x_int -/+ U[0, 1] around a point x_int strictly inside every
+    cone.  F: p = n / 6 rows, 2M nnz N(0, 1), grouped into affine SOC blocks
+    of U{3..64} rows; g = s0 - F x_int with s0 = (0, ..., 0, 2) per block, so
+    x_int is strictly feasible for the affine cones too.  Q = B'B + diag(U[0.1,
+    1]) with B of n / 30 rows and 10 entries per row (nnz(Q) about 2M at
+    scale 1); the diagonal part makes the objective strongly convex, so the
+    problem is bounded for any c.  c ~ N(0, 1).  Cone slots are free.
+    """
+    rng = np.random.default_rng(seed)
+    n_exp = max(1, int(round(33_333 * scale)))
+    n_pow = max(1, int(round(16_667 * scale)))
+    n_psd = max(1, int(round(67 * scale)))
+    orders = rng.integers(psd_orders[0], psd_orders[1] + 1, size=n_psd)
+    fixed_slots = 3 * n_exp + 3 * n_pow + int(sum(k * (k + 1) // 2 for k in orders))
+    n_target = fixed_slots / 0.6
+    blocks = [("exp", 1, 0.0)] * n_exp + [("pow", 1, float(a)) for a in rng.uniform(0.1, 0.9, size=n_pow)]
+    blocks += [("psd", int(k), 0.0) for k in orders]
+    soc_slots = 0
+    while soc_slots < 0.3 * n_target:
+        size = int(rng.integers(3, 65))
+        blocks.append(("soc", size - 2, 0.0))
+        soc_slots += size
+    rsoc_slots = 0
+    while rsoc_slots < 0.1 * n_target:
+        size = int(rng.integers(3, 33))
+        blocks.append(("rsoc", size - 2, 0.0))
+        rsoc_slots += size
+    order = rng.permutation(len(blocks))
+    kinds = {"soc": capi.CONE_STANDARD_SOC, "rsoc": capi.CONE_ROTATED_SOC, "exp": capi.CONE_EXPONENTIAL,
+             "pow": capi.CONE_POWER, "psd": capi.CONE_PSD}
+    slots = {"soc": lambda v: v + 2, "rsoc": lambda v: v + 2, "exp": lambda v: 3, "pow": lambda v: 3,
+             "psd": lambda v: v * (v + 1) // 2}
+    n = int(sum(slots[blocks[i][0]](blocks[i][1]) for i in order))
+    xi = np.empty(n)
+    cones = []
+    pos = 0
+    for i in order:
+        kind, vd, alpha = blocks[i]
+        L = slots[kind](vd)
+        if kind == "soc":
+            v = rng.standard_normal(vd + 1)
+            xi[pos:pos + vd + 1] = v
+            xi[pos + vd + 1] = np.linalg.norm(v) + 1.0
+        elif kind == "rsoc":
+            v = rng.standard_normal(vd)
+            xi[pos:pos + vd] = v
+            xi[pos + vd] = xi[pos + vd + 1] = np.sqrt(0.5 * v @ v) + 1.0
+        elif kind == "exp":
+            xi[pos:pos + 3] = [0.0, 1.0, 2.0]
+        elif kind == "pow":
+            xi[pos:pos + 3] = [1.0, 1.0, 0.3]
+        else:
+            M = rng.standard_normal((vd, vd)) / np.sqrt(vd)
+            S = M @ M.T + np.eye(vd)
+            r, c = np.tril_indices(vd)
+            order_cm = np.lexsort((r, c))  # svec: lower triangle, column-major
+            vals = S[r[order_cm], c[order_cm]] * np.where(r[order_cm] == c[order_cm], 1.0, np.sqrt(2.0))
+            xi[pos:pos + L] = vals
+        cones.append((kinds[kind], pos, vd, alpha) if kind == "pow" else (kinds[kind], pos, vd))
+        pos += L
+    nnz_a = max(n, int(round(2_000_000 * scale)))
+    m = max(2, n // 4)
+    A = random_sparse(rng, m, n, min(nnz_a, m * n))
+    ax = A @ xi
+    kb = max(2, n // 30)
+    B = sp.csr_matrix((rng.standard_normal(10 * kb), (np.repeat(np.arange(kb), 10), rng.integers(0, n, 10 * kb))),
+                      shape=(kb, n))
+    Q = (B.T @ B + sp.diags(rng.uniform(0.1, 1.0, n))).tocsr()
+    Q.sort_indices()
+    data = dict(c=rng.standard_normal(n), A=A, Q=Q, con_lb=ax - rng.uniform(0, 1, m),
+                con_ub=ax + rng.uniform(0, 1, m), var_lb=np.full(n, -np.inf), var_ub=np.full(n, np.inf),
+                cones=cones)
+    if with_affine:
+        p_target = max(3, n // 6)
+        acones, rows = [], 0
+        while rows < p_target:
+            size = int(rng.integers(3, 65))
+            acones.append((capi.CONE_STANDARD_SOC, rows, size - 2))
+            rows += size
+        F = random_sparse(rng, rows, n, min(max(rows, int(round(2_000_000 * scale))), rows * n))
+        s0 = np.zeros(rows)
+        for (_, st, vd) in acones:
+            s0[st + vd + 1] = 2.0
+        data.update(F=F, g=s0 - F @ xi, affine_cones=acones)
+    return QPData(**data)

@@ -86,16 +86,43 @@ def _mat_to_svec(M):
     return np.array([M[r, c] if r == c else M[r, c] * np.sqrt(2) for c in range(k) for r in range(c, k)])
 
 
-@pytest.mark.parametrize("k", [2, 5, 16, 33])
-def test_psd_matches_numpy_eigh(k):
-    rng = np.random.default_rng(k)
-    B = rng.standard_normal((k, k))
-    M = (B + B.T) / 2
+def psd_test_matrix(k, rng, kind):
+    """Symmetric test matrices: 'random' (B + B')/2; 'clustered' a random basis
+    with eigenvalues in tight clusters (spacings 1e-12 .. 1e-9) on both sides
+    of zero and a few within 1e-14 of zero; 'rank1' a negative-definite
+    matrix plus one large positive direction."""
+    if kind == "random":
+        B = rng.standard_normal((k, k))
+        return (B + B.T) / 2
+    Qm, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    if kind == "clustered":
+        base = rng.choice([-2.0, -1.0, 1.0, 3.0], size=k)
+        lam = base + rng.choice([0.0, 1e-12, 1e-10, 1e-9], size=k)
+        lam[: max(1, k // 10)] = rng.uniform(-1e-14, 1e-14, size=max(1, k // 10))
+    else:
+        lam = -rng.uniform(0.5, 1.5, size=k)
+        lam[0] = 50.0
+    return (Qm * lam) @ Qm.T
+
+
+@pytest.mark.parametrize("k", [2, 5, 16, 33, 64, 97, 128])
+@pytest.mark.parametrize("kind", ["random", "clustered", "rank1"])
+def test_psd_matches_numpy_eigh(k, kind):
+    # the oracle's cyclic Jacobi against LAPACK's symmetric eigensolver, to
+    # the orders of the conic-mix workload (16 .. 128); the projection
+    # sum max(w, 0) v v' is unique even where eigenvalues repeat
+    rng = np.random.default_rng(k + 1000 * len(kind))
+    M = psd_test_matrix(k, rng, kind)
     x = _mat_to_svec(M)
     w, V = np.linalg.eigh(M)
     want = _mat_to_svec((V * np.maximum(w, 0)) @ V.T)
     got = project(capi.CONE_PSD, x, k)
-    assert np.max(np.abs(got - want)) <= 1e-12 * (1 + np.max(np.abs(want)))
+    assert np.max(np.abs(got - want)) <= 1e-12 * (1 + np.max(np.abs(x)))
+    # Moreau: x = P(x) - P(-x), with P(x) and P(-x) orthogonal and PSD
+    neg = project(capi.CONE_PSD, -x, k)
+    assert np.max(np.abs(got - neg - x)) <= 1e-12 * (1 + np.max(np.abs(x)))
+    assert abs(got @ neg) <= 1e-11 * (1 + x @ x)
+    assert np.linalg.eigvalsh(_svec_to_mat(got, k)).min() >= -1e-12 * (1 + np.max(np.abs(x)))
 
 
 def in_soc(p, tol=1e-10):

@@ -10,6 +10,7 @@ library and the oracle must agree bit for bit (same rand() draws after the
 same srand()).
 """
 import ctypes as C
+import os
 import itertools
 
 import numpy as np
@@ -315,3 +316,16 @@ def test_permuted_solve_maps_back():
     res.primal_solution, res.dual_solution, res.reduced_cost = capi.as_ptr(xb), capi.as_ptr(yb), capi.as_ptr(rb)
     L.repermute_solution(C.byref(res), _iarr(rp), _iarr(cp))
     assert np.allclose(xb, s0.x, atol=1e-5)
+
+
+def test_reference_partition_client():
+    # the reference's own test/test_partition_utils.c (known answers of
+    # optimize_partition_cuts), compiled unchanged by oracle/Makefile
+    # (ref-clients) and linked to this library; host code only
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "test_partition_utils")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built (needs the reference tree at build time)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
