@@ -403,14 +403,21 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         Cone *d_cones = (Cone *)track(upload(cones.get(), (size_t)coff[B]));
         State *d_states = (State *)track(upload(states));
         Out *d_out = (Out *)track(upload(std::vector<Out>((size_t)B)));
-        hipStream_t st;
-        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        // stream owned by a guard, so it is destroyed on every exit
+        struct StreamGuard {
+            hipStream_t s = nullptr;
+            ~StreamGuard() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } sg;
+        HIP_OK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+        hipStream_t st = sg.s;
         const auto t_dev = std::chrono::steady_clock::now();
         int windows = 0;
         // windows of iterations; every problem that has terminated returns at once
         const int window = std::max(1, std::min(dp.ef * 16, 2000));
         std::vector<Out> out((size_t)B);
-        bool time_up = false;
+        bool time_up = false, cancelled = false;
         for (;;) {
             hipLaunchKernelGGL(batch_solve_kernel, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool, d_vpool,
                                d_slabs, d_cones, d_states, d_out, window);
@@ -421,6 +428,10 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             bool all = true;
             for (const Out &o : out) all = all && o.status != TERMINATION_REASON_UNSPECIFIED;
             if (all) break;
+            if (g_pdhcg_cancel) {  // user interrupt, checked between windows
+                cancelled = true;
+                break;
+            }
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (el >= P.termination_criteria.time_sec_limit) {
                 time_up = true;
@@ -434,7 +445,6 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                          std::chrono::duration<double>(t_dev - t0).count(),
                          std::chrono::duration<double>(t_end - t_dev).count(), windows);
         HIP_OK(hipMemcpy(slabs.get(), d_slabs, (size_t)soff[B] * sizeof(double), hipMemcpyDeviceToHost));
-        (void)hipStreamDestroy(st);
         for (int b = 0; b < B; ++b) {
             const Inst &I = insts[b];
             const ScaledProblem &sp = pk[b].sp;
@@ -478,7 +488,9 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             r->dual_ray_objective = o.dray_obj;
             r->termination_reason = o.status != TERMINATION_REASON_UNSPECIFIED
                                         ? (termination_reason_t)o.status
-                                        : (time_up ? TERMINATION_REASON_TIME_LIMIT : TERMINATION_REASON_UNSPECIFIED);
+                                        : cancelled ? TERMINATION_REASON_USER_INTERRUPT
+                                        : time_up   ? TERMINATION_REASON_TIME_LIMIT
+                                                    : TERMINATION_REASON_UNSPECIFIED;
             results[idx[b]] = r;
         }
         for (void *p : dev) (void)hipFree(p);

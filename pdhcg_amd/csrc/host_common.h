@@ -55,4 +55,8 @@ int validate_fixed_sections(const qp_problem_t *problem);
 qp_problem_t *lift_quadratic_constraints(const qp_problem_t *problem, cone_type_t form);
 void restore_result_dimensions(pdhcg_result_t *result, const qp_problem_t *original);
 
+// Cancellation request set by pdhcg_request_cancel (reference src/pdhcg.c:31);
+// defined in solver.hip, read by every solve loop between evaluations.
+extern volatile int g_pdhcg_cancel;
+
 }  // namespace pdhcg

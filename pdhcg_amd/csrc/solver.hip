@@ -514,10 +514,15 @@ template <class V, class X> class Solver final : public SolverBase {
     // relative KKT error near 1e-6 on problems whose |A||x| is large next to
     // the right-hand sides).  When the FP64-evaluated error has not halved
     // for stall_iters_ iterations, the state moves to the evaluation copy,
-    // which finishes the solve with FP64 iterates (promoted_).
+    // which finishes the solve with FP64 iterates (promoted_).  The inner
+    // BB solve meets the same floor sooner: its tolerance follows the primal
+    // movement down to 1e-9, below what FP32 steps can resolve, and every
+    // inner solve then runs to the inner iteration limit.  kSatPromote such
+    // solves in a row also hand over to FP64.
+    static constexpr int kSatPromote = 4;
     bool promoted_ = false;
     double best_err_ = INFINITY;
-    int best_err_iter_ = 0, stall_iters_ = 1000;
+    int best_err_iter_ = 0, stall_iters_ = 1000, sat_run_ = 0;
     bool stalled(int tc);
     void promote();
     int n_ = 0, m_ = 0, k_ = 0;
@@ -1185,7 +1190,7 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
     // drains while the host waits: when the polled flag says the loop ended,
     // the queued chunk runs gated (empty launches).  The first chunk is the
     // body count of the previous solve.
-    int issued = 0, inflight = 0, head = 0, chunk = std::max(1, bb_pred_);
+    int issued = 0, inflight = 0, head = 0, chunk = std::max(1, bb_pred_), last_iter = 0;
     auto enqueue_chunk = [&](int count) {
         count = std::min(count, max_bodies - issued);
         for (int c = 0; c < count; ++c) enqueue_body();
@@ -1203,6 +1208,7 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
         }
         HIP_OK(hipEventSynchronize(bb_ev_[head]));
         const int active = bb_poll_[2 * head], iter = bb_poll_[2 * head + 1];
+        last_iter = iter;
         head ^= 1;
         --inflight;
         if (!active) {
@@ -1211,6 +1217,7 @@ template <class V, class X> void Solver<V, X>::bb_prox(double tau, bool fuse_hal
         }
         if (inflight == 0) chunk = std::max(chunk, 2);
     }
+    if (eval_) sat_run_ = (inner_limit_ > 1 && last_iter >= inner_limit_) ? sat_run_ + 1 : 0;
     hipLaunchKernelGGL(bb_final_kernel<X>, dim3(nb), dim3(kBlock), 0, st_, x_, xp_, xr_, x0_, n_,
                        fuse_halpern ? 1 : 0, w, P_.reflection_coefficient, ds_);
 }
@@ -1743,6 +1750,13 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
             }
         }
         const int tc = total_count_;
+        if (eval_ && stall_iters_ > 0 && sat_run_ >= kSatPromote) {
+            promote();
+            if (P_.verbose >= 1)
+                std::printf("[mixed] %d FP32 inner solves in a row reached the inner limit; FP64 iterates from "
+                            "iteration %d\n", sat_run_, tc);
+            return eval_->run(iterations - done);
+        }
         if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
             residual();
             if (!has_cones_ && !has_affine_ && is_major_ && tc < 3 * ef) infeasibility();

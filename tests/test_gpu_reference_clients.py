@@ -19,7 +19,8 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 CLIENTS = ["test_c_api_working", "test_affine_cones", "test_cone_box_reject", "test_conic_e2e",
            "test_conic_kkt_termination", "test_diag_q_cones", "test_fisher", "test_initial_box_projection",
            "test_power_cone", "test_problem_validation", "test_sparse_q_cones", "test_qcqp_isfixed",
-           "test_qcqp_no_aux", "test_qcqp_rsoc", "test_qcqp_scale", "test_qcqp_sep"]
+           "test_qcqp_no_aux", "test_qcqp_rsoc", "test_qcqp_scale", "test_qcqp_sep", "test_cbf_affine_cones",
+           "test_cbf_fixed_slots"]
 
 
 @pytest.mark.parametrize("name", CLIENTS)
