@@ -94,6 +94,22 @@ int pdhcg_solver_kernel_time(pdhcg_solver_t *solver, int kernel, double *total_m
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n);
 int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n);
 
+/* pdhcg_project_cones without device allocation; it has no counterpart in the
+ * reference.  It is for callers that cannot call hipMalloc / hipFree: code
+ * under HIP graph capture elsewhere in the process, tight serving loops,
+ * harnesses that forbid device allocation.  Same arguments, kernels and
+ * results (bit-identical) as pdhcg_project_cones, but every device buffer of
+ * the call (the point, the scaling, the block lists and their warm starts, the
+ * PSD and long-section work space) is carved from one fixed 32 MiB array of
+ * the library in each device's global memory, held until the process ends.
+ * A call whose buffers exceed it (about n <= 1.9M slots with few blocks) is
+ * refused: -1 and a message, x untouched.  Calls from several host threads
+ * are serialized by a process-wide lock (they would otherwise race on the
+ * one array); the array keeps the last call's contents, and each call
+ * clears what must start at zero.  Works on the device current at the call.
+ * Returns 0, or -1 with a message on stderr. */
+int pdhcg_project_cones_noalloc(const cone_blocks_t *cones, const double *d, double *x, int n);
+
 /* Batched solve; the reference solves one problem per call and has no
  * counterpart.  `count` independent problems are advanced together on the
  * device, one workgroup per problem, each running the same restarted Halpern

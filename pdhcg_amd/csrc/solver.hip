@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -91,6 +92,22 @@ template <class T> T *dzeros(size_t count) {
     return p;
 }
 
+// Bump allocation from a caller-provided device region (the cone set of the
+// allocation-free entry point).  With base == nullptr it only measures: the
+// returned offsets are never dereferenced and no copies are issued.
+struct DevArena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool measuring() const { return base == nullptr; }
+    template <class T> T *take(size_t count) {
+        const size_t bytes = ((std::max<size_t>(count, 1) * sizeof(T)) + 255) & ~(size_t)255;
+        if (used + bytes > cap) fail("device scratch of %zu bytes exhausted (%zu more needed)", cap, used + bytes - cap);
+        T *p = reinterpret_cast<T *>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
 template <class V> BinnedCsr<V> upload_csr(const HostCSR &h) {
     return make_binned<V>(h.rows, h.cols, h.rp, h.ci, h.v);
 }
@@ -159,7 +176,10 @@ class DevConeSet {
     DevConeSet &operator=(const DevConeSet &) = delete;
     ~DevConeSet() { release(); }
 
-    void build(const cone_blocks_t *b, int n) {
+    // arena: place every array in that region instead of hipMalloc (nothing
+    // is freed by release() then)
+    void build(const cone_blocks_t *b, int n, DevArena *arena = nullptr) {
+        arena_ = arena;
         struct HostList {
             std::vector<int> start, vdim;
             std::vector<double> alpha;
@@ -222,37 +242,37 @@ class DevConeSet {
             ConeList &L = *dst[li];
             L.count = (int)h.start.size();
             if (!L.count) continue;
-            L.start = dupload<int>(h.start, h.start.size());
-            L.vdim = dupload<int>(h.vdim, h.vdim.size());
-            L.alpha = dupload<double>(h.alpha, h.alpha.size());
-            L.warm = dzeros<double>(h.start.size());
-            L.rwarm = dzeros<double>(h.start.size());
+            L.start = up<int>(h.start, h.start.size());
+            L.vdim = up<int>(h.vdim, h.vdim.size());
+            L.alpha = up<double>(h.alpha, h.alpha.size());
+            L.warm = zeros<double>(h.start.size());
+            L.rwarm = zeros<double>(h.start.size());
         }
         if (!pall_start.empty()) {
             powall.count = (int)pall_start.size();
-            powall.start = dupload<int>(pall_start, pall_start.size());
-            powall.alpha = dupload<double>(pall_alpha, pall_alpha.size());
+            powall.start = up<int>(pall_start, pall_start.size());
+            powall.alpha = up<double>(pall_alpha, pall_alpha.size());
         }
         if (!sec_start.empty()) {
             sec.count = (int)sec_start.size();
-            sec.start = dupload<int>(sec_start, sec_start.size());
-            sec.vdim = dupload<int>(sec_vdim, sec_vdim.size());
-            sec.type = dupload<int>(sec_type, sec_type.size());
-            sec.alpha = dupload<double>(sec_alpha, sec_alpha.size());
+            sec.start = up<int>(sec_start, sec_start.size());
+            sec.vdim = up<int>(sec_vdim, sec_vdim.size());
+            sec.type = up<int>(sec_type, sec_type.size());
+            sec.alpha = up<double>(sec_alpha, sec_alpha.size());
         }
         if (!lsec_start.empty()) {
             lsec.count = (int)lsec_start.size();
-            lsec.start = dupload<int>(lsec_start, lsec_start.size());
-            lsec.vdim = dupload<int>(lsec_vdim, lsec_vdim.size());
-            lsec.type = dupload<int>(lsec_type, lsec_type.size());
+            lsec.start = up<int>(lsec_start, lsec_start.size());
+            lsec.vdim = up<int>(lsec_vdim, lsec_vdim.size());
+            lsec.type = up<int>(lsec_type, lsec_type.size());
             std::vector<double> zero_alpha(lsec_start.size(), 0.0);
-            lsec.alpha = dupload<double>(zero_alpha, zero_alpha.size());
-            lsec_off = dupload<long long>(lsec_off_h, lsec_off_h.size());
-            lsec_scratch = dalloc<double>((size_t)lsec_total);
+            lsec.alpha = up<double>(zero_alpha, zero_alpha.size());
+            lsec_off = up<long long>(lsec_off_h, lsec_off_h.size());
+            lsec_scratch = raw<double>((size_t)lsec_total);
         }
         if (!sec_start.empty() || !lsec_start.empty()) {
             std::vector<char> hf(b->is_fixed, b->is_fixed + n);
-            fixed = dupload<char>(hf, hf.size());
+            fixed = up<char>(hf, hf.size());
         }
         std::sort(ps.begin(), ps.end());
         for (size_t i = 0; i < ps.size();) {
@@ -262,7 +282,7 @@ class DevConeSet {
             PsdGroup g;
             g.order = ps[i].first;
             g.count = (int)starts.size();
-            g.starts = dupload<int>(starts, starts.size());
+            g.starts = up<int>(starts, starts.size());
             const size_t budget = psd_lds_budget();
             g.mode = psd_lds_bytes(g.order, 0) <= budget ? 0 : (psd_lds_bytes(g.order, 1) <= budget ? 1 : 2);
             // PDHCG_PSD_MODE=1|2 moves V' (1) or A and V' (2) to the work buffer
@@ -270,14 +290,14 @@ class DevConeSet {
             if (const char *fm = std::getenv("PDHCG_PSD_MODE")) g.mode = std::max(g.mode, std::min(2, std::atoi(fm)));
             g.lds = psd_lds_bytes(g.order, g.mode);
             const size_t wd = psd_work_doubles(g.order, g.mode) * (size_t)g.count;
-            if (wd) g.work = dalloc<double>(wd);
+            if (wd) g.work = raw<double>(wd);
             groups.push_back(g);
             i = j;
         }
-        mask = dupload<char>(hmask, hmask.size());
+        mask = up<char>(hmask, hmask.size());
         total = (int)hs.size();
-        all_start = dupload<int>(hs, hs.size());
-        all_len = dupload<int>(hl, hl.size());
+        all_start = up<int>(hs, hs.size());
+        all_len = up<int>(hl, hl.size());
     }
 
     // Projects v onto the scaled cones D K block by block (reference
@@ -383,6 +403,10 @@ class DevConeSet {
     }
 
     void release() {
+        if (arena_) {  // arena memory belongs to the caller
+            *this = DevConeSet::Cleared();
+            return;
+        }
         for (ConeList *L : {&soc, &rsoc, &expc, &powc, &lsoc, &lrsoc, &powall}) {
             void *lp[] = {L->start, L->vdim, L->alpha, L->warm, L->rwarm};
             for (void *p : lp)
@@ -413,6 +437,39 @@ class DevConeSet {
 
   private:
     static dim3 blocks(int count) { return dim3((unsigned)((count + kBlock - 1) / kBlock)); }
+
+    DevArena *arena_ = nullptr;
+    struct Cleared {};
+    DevConeSet &operator=(Cleared) {
+        psd = power = false;
+        rsoc = soc = expc = powc = lrsoc = lsoc = powall = ConeList();
+        sec = lsec = SectionList();
+        lsec_scratch = nullptr;
+        lsec_off = nullptr;
+        fixed = mask = nullptr;
+        groups.clear();
+        total = 0;
+        all_start = all_len = nullptr;
+        arena_ = nullptr;
+        return *this;
+    }
+    template <class T, class S> T *up(const std::vector<S> &h, size_t count) {
+        if (!arena_) return dupload<T>(h, count);
+        T *p = arena_->take<T>(count);
+        if (count && !arena_->measuring()) {
+            std::vector<T> tmp(count);
+            for (size_t i = 0; i < count; ++i) tmp[i] = (T)h[i];
+            HIP_OK(hipMemcpy(p, tmp.data(), count * sizeof(T), hipMemcpyHostToDevice));
+        }
+        return p;
+    }
+    template <class T> T *zeros(size_t count) {
+        if (!arena_) return dzeros<T>(count);
+        T *p = arena_->take<T>(count);
+        if (!arena_->measuring()) HIP_OK(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(T)));
+        return p;
+    }
+    template <class T> T *raw(size_t count) { return arena_ ? arena_->take<T>(count) : dalloc<T>(count); }
 };
 
 }  // namespace
@@ -2071,19 +2128,19 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
         std::vector<double> dh((size_t)n, 1.0);
         if (d) std::copy(d, d + n, dh.begin());
         std::vector<double> vh(in, in + n);
-        double *dd = dupload<double>(dh, n), *v = dupload<double>(vh, n), *res = nullptr, *scratch = nullptr,
+        double *dd = dupload<double>(dh, n), *v = dupload<double>(vh, n), *res = nullptr, *psd_buf = nullptr,
                *zero = dzeros<double>(n);
         if (dual) {
             res = dupload<double>(std::vector<double>(out, out + n), n);
-            scratch = dzeros<double>(n);
-            set.dual_residual<double>(res, scratch, zero, v, nullptr, zero, dd, n, st);
+            psd_buf = dzeros<double>(n);
+            set.dual_residual<double>(res, psd_buf, zero, v, nullptr, zero, dd, n, st);
         } else {
             set.project<double>(v, dd, false, nullptr, st);
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out, dual ? res : v, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        for (void *p : {(void *)dd, (void *)v, (void *)res, (void *)scratch, (void *)zero})
+        for (void *p : {(void *)dd, (void *)v, (void *)res, (void *)psd_buf, (void *)zero})
             if (p) (void)hipFree(p);
         set.release();
         (void)hipStreamDestroy(st);
@@ -2097,6 +2154,79 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
 
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n) {
     return cone_entry("pdhcg_project_cones", cones, d, x, x, n, false);
+}
+
+// ---- allocation-free variant of pdhcg_project_cones ----
+//
+// Every device buffer of the call (the point, the scaling, the per-type block
+// lists, their warm starts, the PSD work and long-section scratch) is carved
+// from `scratch`, a fixed array of kNoAllocScratchBytes in each device's
+// global memory for the life of the process.  The host reaches it only
+// through the address hipGetSymbolAddress gives on that device (kept per
+// device in g_scratch_ptr); the array's name is never passed to a kernel.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;                        // one call at a time per process
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};  // device address of `scratch`, per device
+
+__global__ void fill_kernel(double *v, double value, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = value;
+}
+}  // namespace
+
+int pdhcg_project_cones_noalloc(const cone_blocks_t *cones, const double *d, double *x, int n) {
+    const char *name = "pdhcg_project_cones_noalloc";
+    if (!cones || !x || n < 0 || cones->num_cones < 0 || (cones->num_cones > 0 && !cones->start_idx)) {
+        std::fprintf(stderr, "[interface] %s: invalid arguments.\n", name);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    try {
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kNoAllocMaxDevices) fail("device %d is beyond the %d the scratch table covers", dev,
+                                                       kNoAllocMaxDevices);
+        if (!g_scratch_ptr[dev]) {
+            void *ptr = nullptr;
+            const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (rc != hipSuccess || !ptr)
+                fail("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(rc));
+            g_scratch_ptr[dev] = ptr;
+        }
+        // size: the point, the scaling and the cone set, measured before anything is written
+        DevArena need;
+        need.cap = ~(size_t)0;
+        need.take<double>((size_t)n);
+        need.take<double>((size_t)n);
+        {
+            DevConeSet probe;
+            probe.build(cones, n, &need);
+        }
+        if (need.used > kNoAllocScratchBytes)
+            fail("needs %zu bytes of device scratch; the fixed scratch holds %zu (n = %d, %d cone blocks)", need.used,
+                 kNoAllocScratchBytes, n, cones->num_cones);
+        DevArena arena;
+        arena.base = static_cast<char *>(g_scratch_ptr[dev]);
+        arena.cap = kNoAllocScratchBytes;
+        double *v = arena.take<double>((size_t)n), *dd = arena.take<double>((size_t)n);
+        if (n > 0) {
+            HIP_OK(hipMemcpy(v, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+            if (d) HIP_OK(hipMemcpy(dd, d, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+            else hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, 0, dd, 1.0, n);
+        }
+        DevConeSet set;
+        set.build(cones, n, &arena);
+        set.project<double>(v, dd, false, nullptr, 0);
+        HIP_OK(hipGetLastError());
+        if (n > 0) HIP_OK(hipMemcpy(x, v, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipDeviceSynchronize());
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[interface] %s: %s\n", name, e.what());
+        return -1;
+    }
 }
 
 int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n) {
