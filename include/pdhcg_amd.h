@@ -94,6 +94,17 @@ int pdhcg_solver_kernel_time(pdhcg_solver_t *solver, int kernel, double *total_m
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n);
 int pdhcg_cone_dual_residual(const cone_blocks_t *cones, const double *d, const double *rc, double *out, int n);
 
+/* Projection of `count` points, stored one after another in x (count * n
+ * doubles), in order and in place, through one device cone set: each PSD
+ * eigensolve starts from the eigenvectors the previous point's projection
+ * left for that block, as the solver's iterations do (reference: the same
+ * projection per point, src/kernels/pdhcg_psd_cone_kernels.cu:415-486, with
+ * a fresh eigensolve each time).  The results are the projections of the
+ * points (the nearest point is unique); only the rounding and the number of
+ * Jacobi sweeps depend on the previous point.  Return 0, or -1 with a
+ * message on stderr. */
+int pdhcg_project_cones_seq(const cone_blocks_t *cones, const double *d, double *x, int n, int count);
+
 /* pdhcg_project_cones without device allocation; it has no counterpart in the
  * reference.  It is for callers that cannot call hipMalloc / hipFree: code
  * under HIP graph capture elsewhere in the process, tight serving loops,

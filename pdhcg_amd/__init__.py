@@ -64,6 +64,9 @@ def lib() -> C.CDLL:
                                         P(P(capi.pdhcg_result_t))]
         L.pdhcg_project_cones.restype = C.c_int
         L.pdhcg_project_cones.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p, C.c_int]
+        L.pdhcg_project_cones_seq.restype = C.c_int
+        L.pdhcg_project_cones_seq.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p, C.c_int,
+                                              C.c_int]
         L.pdhcg_cone_dual_residual.restype = C.c_int
         L.pdhcg_cone_dual_residual.argtypes = [P(capi.cone_blocks_t), capi.c_double_p, capi.c_double_p,
                                                capi.c_double_p, C.c_int]
@@ -125,6 +128,19 @@ def project_cones(specs, x, d=None, fixed=None) -> np.ndarray:
     b, _keep = capi.cone_blocks(specs, fixed)
     if L.pdhcg_project_cones(C.byref(b), capi.as_ptr(dd), capi.as_ptr(out), len(out)) != 0:
         raise RuntimeError("pdhcg_project_cones failed")
+    return out
+
+
+def project_cones_seq(specs, xs, d=None) -> np.ndarray:
+    """Device projections of the rows of ``xs`` (count x n) in order through
+    one cone set (pdhcg_project_cones_seq); returns a new array."""
+    L = lib()
+    out = np.ascontiguousarray(np.array(xs, dtype=np.float64, copy=True))
+    count, n = out.shape
+    dd = None if d is None else capi.carray(d)
+    b, _keep = capi.cone_blocks(specs)
+    if L.pdhcg_project_cones_seq(C.byref(b), capi.as_ptr(dd), capi.as_ptr(out), n, count) != 0:
+        raise RuntimeError("pdhcg_project_cones_seq failed")
     return out
 
 

@@ -776,38 +776,62 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
 // ---------------- PSD cone: one workgroup per matrix ----------------
 //
 // Parallel two-sided Jacobi (Golub & Van Loan, sec. 8.5) on the dense
-// symmetric matrix of an svec block.  The order k is padded to an even kp
-// (a zero row and column: a zero eigenvalue that contributes nothing).  A
-// sweep is kp - 1 rounds of the circle (round-robin tournament) ordering; the
-// kp / 2 pairs of a round are disjoint, so their rotations are applied at
+// symmetric matrix of an svec block, every PSD block of the problem in one
+// launch (one workgroup per block, largest orders first).  The order k is
+// padded to an even kp (a zero row and column: a zero eigenvalue that
+// contributes nothing).
+//
+// Warm start: each block keeps the eigenvector rows V' of its previous
+// projection (identity before the first) in its slice of the work buffer.
+// The sweeps start from B = V' A V'^T with V' as the initial rotation, so a
+// matrix close to the previous one is nearly diagonal already and needs two
+// or three sweeps instead of six to nine.  The result is the projection of A
+// whatever V' holds; V' only changes the rounding (the nearest PSD matrix is
+// unique).  A projection that does not converge resets V' to the identity.
+//
+// A sweep is kp - 1 rounds of the circle (round-robin tournament) ordering;
+// the kp / 2 pairs of a round are disjoint, so their rotations are applied at
 // once.  A round is two phases separated by barriers:
 //   1. each pair (p, q) gets its rotation (c, s) from its 2x2 diagonal block;
-//   2. every 2x2 block A[{p,q},{r,s}] with pair P <= pair R becomes
-//      J_P' B J_R (its mirror is written from the same values, so A stays
-//      exactly symmetric; a diagonal block's off-diagonal entry is set to 0),
-//      and rows p, q of V' become J_P' V'[{p,q}, :].
-// A lives in LDS; V' too when both fit the workgroup's LDS, otherwise in the
-// matrix's slice of the global work buffer (mode 1); with neither fitting
-// (orders above about 128) both use the work buffer (mode 2).  Sweeps run
-// until the off-diagonal mass is at most 1e-30 of the total.  A matrix not
-// converged after kPsdMaxSweeps sweeps (only non-finite input does that) is
-// written as NaN, as the reference writes NaN when its eigensolver fails
-// (reference src/kernels/pdhcg_psd_cone_kernels.cu:263-486).
+//   2. every 2x2 block A[{p,q},{r,s}] (pair P, pair R) becomes J_P' B J_R,
+//      computed by the thread that owns the block in the orientation
+//      min(P, R), max(P, R), so the two mirror blocks hold bit-identical
+//      values and A stays exactly symmetric (a diagonal block's off-diagonal
+//      entry is set to 0); rows p, q of V' become J_P' V'[{p,q}, :].
+// A lives in LDS; V' too when both fit the workgroup's LDS (mode 0),
+// otherwise V' stays in the work buffer (mode 1); with neither fitting
+// (orders above about 140) both do (mode 2).  Sweeps run until the
+// off-diagonal mass is at most 1e-30 of the total.  A matrix not converged
+// after kPsdMaxSweeps sweeps (only non-finite input does that) is written as
+// NaN, as the reference writes NaN when its eigensolver fails (reference
+// src/kernels/pdhcg_psd_cone_kernels.cu:263-486).
 
 constexpr int kPsdMaxSweeps = 60;
-constexpr int kPsdThreads = 256;
+constexpr int kPsdThreads = 512;
 
-// Workgroup LDS (dynamic) of a projection of order k in a mode.
+// One PSD block of a launch.
+struct PsdBlock {
+    int start;       // first svec slot in v
+    int order;       // k
+    int mode;        // where A and V' live during the sweeps (see above)
+    int pad;
+    long long work;  // offset (doubles) of the block's slice of the work buffer
+};
+
+// Workgroup LDS (dynamic) of a projection of order k in a mode: A and V'
+// (by mode), the round's rotations and pairs, and the clamped eigenvalues.
 __host__ __device__ inline size_t psd_lds_bytes(int k, int mode) {
     const size_t kp = (size_t)(k + (k & 1)), np = kp / 2;
     const size_t mats = mode == 0 ? 2 : (mode == 1 ? 1 : 0);
-    return (mats * kp * kp + 2 * np) * sizeof(double) + 2 * np * sizeof(int);
+    return (mats * kp * kp + 2 * np + kp) * sizeof(double) + 2 * np * sizeof(int);
 }
 
-// Global work doubles per matrix in a mode.
+// Work doubles per block: the kept V' of the two projection sites (the
+// iterate and the dual residual), the transform's temporary and, in mode 2,
+// A itself.
 __host__ __device__ inline size_t psd_work_doubles(int k, int mode) {
     const size_t kp = (size_t)(k + (k & 1));
-    return mode == 0 ? 0 : (mode == 1 ? kp * kp : 2 * kp * kp);
+    return (mode == 2 ? 4 : 3) * kp * kp;
 }
 
 // Row r and column c (r >= c) of svec slot `slot` of an order-k block
@@ -822,15 +846,53 @@ __device__ __forceinline__ void psd_slot_rc(int slot, int k, int &r, int &c) {
     r = c + rem;
 }
 
-template <class X, int Mode>
-__global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const int *starts, int k, double *work,
-                                                                   const int *gate) {
+// Sets the kept V' of every block of a launch to the identity.
+static __global__ void psd_identity_kernel(const PsdBlock *blocks, double *work) {
+    const PsdBlock b = blocks[blockIdx.x];
+    const size_t kp = (size_t)(b.order + (b.order & 1));
+    double *V = work + b.work;
+    for (size_t i = threadIdx.x; i < 2 * kp * kp; i += blockDim.x) {
+        const size_t e = i % (kp * kp);
+        V[i] = (e / kp == e % kp) ? 1.0 : 0.0;
+    }
+}
+
+// 4 x 4 register tile of C = X Y over kp x kp row-major operands, rows
+// i0.., columns j0..; entries outside kp read as zero.  Lanes of a wave share
+// the tile row (X reads broadcast) and take consecutive column tiles (Y reads
+// contiguous).
+__device__ __forceinline__ void psd_tile_xy(const double *X, const double *Y, int kp, int i0, int j0,
+                                            double acc[4][4]) {
+    for (int u = 0; u < 4; ++u)
+        for (int w = 0; w < 4; ++w) acc[u][w] = 0.0;
+    const double *xr[4];
+    bool xi[4], yj[4];
+    for (int u = 0; u < 4; ++u) {
+        xi[u] = i0 + u < kp;
+        yj[u] = j0 + u < kp;
+        xr[u] = X + (size_t)(xi[u] ? i0 + u : i0) * kp;
+    }
+    for (int l = 0; l < kp; ++l) {
+        const double *yl = Y + (size_t)l * kp + j0;
+        double a[4], b[4];
+        for (int u = 0; u < 4; ++u) {
+            a[u] = xi[u] ? xr[u][l] : 0.0;
+            b[u] = yj[u] ? yl[u] : 0.0;
+        }
+        for (int u = 0; u < 4; ++u)
+            for (int w = 0; w < 4; ++w) acc[u][w] = fma(a[u], b[w], acc[u][w]);
+    }
+}
+
+template <class X>
+__global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const PsdBlock *blocks, double *work,
+                                                                   int site, const int *gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double psd_lds[];
     __shared__ double red[2 * (kPsdThreads / kWave)];
     __shared__ int stop;
-    const int mat = blockIdx.x;
-    const int start = starts[mat];
+    const PsdBlock blk = blocks[blockIdx.x];
+    const int start = blk.start, k = blk.order, mode = blk.mode;
     const int tid = threadIdx.x, nt = blockDim.x;
     if (k == 1) {
         if (tid == 0) v[start] = (X)fmax((double)v[start], 0.0);
@@ -838,17 +900,20 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const in
     }
     const int kp = k + (k & 1), np = kp / 2, m = kp - 1;
     const size_t kk = (size_t)kp * kp;
-    double *wbase = work + (size_t)mat * psd_work_doubles(k, Mode);
-    double *A = Mode == 2 ? wbase : psd_lds;
-    double *Vt = Mode == 0 ? psd_lds + kk : (Mode == 1 ? wbase : wbase + kk);
-    double *cs = psd_lds + (Mode == 0 ? 2 * kk : (Mode == 1 ? kk : 0));
-    int *pq = reinterpret_cast<int *>(cs + 2 * np);
+    double *wb = work + blk.work;
+    double *Vkeep = wb + (size_t)site * kk;  // V' kept between calls at this site
+    double *S = wb + 2 * kk;                  // temporary of the warm transform
+    double *A = mode == 2 ? wb + 3 * kk : psd_lds;
+    double *Vt = mode == 0 ? psd_lds + kk : Vkeep;
+    double *cs = psd_lds + (mode == 0 ? 2 * kk : (mode == 1 ? kk : 0));
+    double *lam = cs + 2 * np;
+    int *pq = reinterpret_cast<int *>(lam + kp);
     const int L = k * (k + 1) / 2;
 
-    // svec -> dense (off-diagonal entries / sqrt 2), zero padding, V' = I
+    // svec -> dense (off-diagonal entries / sqrt 2) with zero padding; V' in
     for (size_t i = tid; i < kk; i += nt) {
         A[i] = 0.0;
-        Vt[i] = (i / kp == i % kp) ? 1.0 : 0.0;
+        if (mode == 0) Vt[i] = Vkeep[i];
     }
     __syncthreads();
     for (int slot = tid; slot < L; slot += nt) {
@@ -858,6 +923,34 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const in
         if (r != c) val *= kInvSqrt2;
         A[(size_t)r * kp + c] = val;
         A[(size_t)c * kp + r] = val;
+    }
+    __syncthreads();
+
+    // warm transform B = V' A V'^T: S = (V' A)^T = A V'^T into the work
+    // buffer, then B = V' S on the lower tiles, mirrored (B is symmetric)
+    const int nb = (kp + 3) / 4;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int ti = t / nb, tj = t - ti * nb;
+        double acc[4][4];
+        psd_tile_xy(Vt, A, kp, 4 * ti, 4 * tj, acc);
+        for (int u = 0; u < 4; ++u)
+            for (int w = 0; w < 4; ++w)
+                if (4 * ti + u < kp && 4 * tj + w < kp) S[(size_t)(4 * tj + w) * kp + 4 * ti + u] = acc[u][w];
+    }
+    __syncthreads();
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int ti = t / nb, tj = t - ti * nb;
+        if (tj > ti) continue;
+        double acc[4][4];
+        psd_tile_xy(Vt, S, kp, 4 * ti, 4 * tj, acc);
+        for (int u = 0; u < 4; ++u)
+            for (int w = 0; w < 4; ++w) {
+                const int i = 4 * ti + u, j = 4 * tj + w;
+                if (i < kp && j <= i) {
+                    A[(size_t)i * kp + j] = acc[u][w];
+                    A[(size_t)j * kp + i] = acc[u][w];
+                }
+            }
     }
     __syncthreads();
 
@@ -877,12 +970,12 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const in
         }
         __syncthreads();
         if (tid == 0) {
-            double o = 0.0, t = 0.0;
+            double o = 0.0, s = 0.0;
             for (int w = 0; w < nt / kWave; ++w) {
                 o += red[2 * w];
-                t += red[2 * w + 1];
+                s += red[2 * w + 1];
             }
-            stop = (o <= 1e-30 * t || o == 0.0) ? 1 : 0;
+            stop = (o <= 1e-30 * s || o == 0.0) ? 1 : 0;
         }
         __syncthreads();
         if (stop) {
@@ -916,33 +1009,35 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const in
                 pq[2 * i + 1] = q;
             }
             __syncthreads();
-            // phase 2: J_P' B J_R on the blocks with P <= R, and V'
+            // phase 2: every 2x2 block by its owner, then the rows of V'
             for (int w = tid; w < np * np; w += nt) {
                 const int P = w / np, R = w - P * np;
-                if (P > R) continue;
                 const int p = pq[2 * P], q = pq[2 * P + 1], r = pq[2 * R], s = pq[2 * R + 1];
-                const double c1 = cs[2 * P], s1 = cs[2 * P + 1], c2 = cs[2 * R], s2 = cs[2 * R + 1];
                 double *Ap = A + (size_t)p * kp, *Aq = A + (size_t)q * kp;
-                const double b00 = Ap[r], b01 = Ap[s], b10 = Aq[r], b11 = Aq[s];
+                const double e00 = Ap[r], e01 = Ap[s], e10 = Aq[r], e11 = Aq[s];
+                if (P == R) {
+                    const double c1 = cs[2 * P], s1 = cs[2 * P + 1];
+                    const double m00 = c1 * e00 - s1 * e10, m01 = c1 * e01 - s1 * e11;
+                    const double m10 = s1 * e00 + c1 * e10, m11 = s1 * e01 + c1 * e11;
+                    Ap[p] = c1 * m00 - s1 * m01;
+                    Aq[q] = s1 * m10 + c1 * m11;
+                    Ap[q] = 0.0;
+                    Aq[p] = 0.0;
+                    continue;
+                }
+                // orientation (lo, hi): lo = min(P, R); b is the (lo, hi) block
+                const bool tr = P > R;
+                const int lo = tr ? R : P, hi = tr ? P : R;
+                const double c1 = cs[2 * lo], s1 = cs[2 * lo + 1], c2 = cs[2 * hi], s2 = cs[2 * hi + 1];
+                const double b00 = e00, b01 = tr ? e10 : e01, b10 = tr ? e01 : e10, b11 = e11;
                 const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
                 const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
                 const double n00 = c2 * m00 - s2 * m01, n01 = s2 * m00 + c2 * m01;
                 const double n10 = c2 * m10 - s2 * m11, n11 = s2 * m10 + c2 * m11;
-                if (P == R) {
-                    Ap[p] = n00;
-                    Aq[q] = n11;
-                    Ap[q] = 0.0;
-                    Aq[p] = 0.0;
-                } else {
-                    Ap[r] = n00;
-                    Ap[s] = n01;
-                    Aq[r] = n10;
-                    Aq[s] = n11;
-                    A[(size_t)r * kp + p] = n00;
-                    A[(size_t)s * kp + p] = n01;
-                    A[(size_t)r * kp + q] = n10;
-                    A[(size_t)s * kp + q] = n11;
-                }
+                Ap[r] = n00;
+                Ap[s] = tr ? n10 : n01;
+                Aq[r] = tr ? n01 : n10;
+                Aq[s] = n11;
             }
             for (int w = tid; w < np * kp; w += nt) {
                 const int P = w / kp, i = w - P * kp;
@@ -956,21 +1051,42 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const in
             __syncthreads();
         }
     }
-    // svec of V max(diag(A), 0) V'
-    for (int slot = tid; slot < L; slot += nt) {
-        int r, c;
-        psd_slot_rc(slot, k, r, c);
-        double val = 0.0;
+    // keep V' for the next call (identity after a failure)
+    if (mode == 0 || !converged)
+        for (size_t i = tid; i < kk; i += nt) Vkeep[i] = converged ? Vt[i] : (i / kp == i % kp ? 1.0 : 0.0);
+    for (int e = tid; e < kp; e += nt) lam[e] = fmax(A[(size_t)e * kp + e], 0.0);
+    __syncthreads();
+    // svec of V' ^T max(diag(A), 0) V' on the lower 4 x 4 tiles of k x k
+    const int nk = (k + 3) / 4;
+    for (int t = tid; t < nk * nk; t += nt) {
+        const int ti = t / nk, tj = t - ti * nk;
+        if (tj > ti) continue;
+        const int r0 = 4 * ti, c0 = 4 * tj;
+        double acc[4][4];
+        for (int u = 0; u < 4; ++u)
+            for (int w = 0; w < 4; ++w) acc[u][w] = 0.0;
         if (converged) {
             for (int e = 0; e < kp; ++e) {
-                const double lam = A[(size_t)e * kp + e];
-                if (lam > 0.0) val += lam * Vt[(size_t)e * kp + r] * Vt[(size_t)e * kp + c];
+                const double le = lam[e];
+                if (le == 0.0) continue;
+                const double *ve = Vt + (size_t)e * kp;
+                double a[4], b[4];
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = r0 + u < k ? le * ve[r0 + u] : 0.0;
+                    b[u] = c0 + u < k ? ve[c0 + u] : 0.0;
+                }
+                for (int u = 0; u < 4; ++u)
+                    for (int w = 0; w < 4; ++w) acc[u][w] = fma(a[u], b[w], acc[u][w]);
             }
-            if (r != c) val *= kSqrt2;
-        } else {
-            val = NAN;
         }
-        v[start + slot] = (X)val;
+        for (int u = 0; u < 4; ++u)
+            for (int w = 0; w < 4; ++w) {
+                const int r = r0 + u, c = c0 + w;
+                if (r >= k || c > r) continue;
+                double val = converged ? acc[u][w] : NAN;
+                if (r != c) val *= kSqrt2;
+                v[start + c * k - c * (c - 1) / 2 + (r - c)] = (X)val;
+            }
     }
 }
 

@@ -120,12 +120,13 @@ struct RunStats {
 // The variable cone blocks on the device: one block list per type, PSD
 // blocks grouped by order, and a slot mask (reference
 // internal/internal_types.h:106-127, dispatch src/cone_dispatch.cu).
-struct PsdGroup {
-    int order = 0, count = 0;
-    int mode = 0;            // where the Jacobi keeps A and V' (psd_project_kernel)
-    size_t lds = 0;          // dynamic LDS bytes per workgroup
-    int *starts = nullptr;
-    double *work = nullptr;  // psd_work_doubles(order, mode) per matrix (null in mode 0)
+// Every PSD block of a cone set: one launch of psd_project_kernel, largest
+// orders first.
+struct PsdSet {
+    int count = 0;
+    size_t lds = 0;             // dynamic LDS bytes per workgroup (the largest need)
+    PsdBlock *blocks = nullptr;
+    double *work = nullptr;     // psd_work_doubles(order, mode) per block
 };
 
 // LDS a workgroup may allocate on the current device, less the PSD kernel's
@@ -137,22 +138,19 @@ inline size_t psd_lds_budget() {
     return lds > 1024 ? (size_t)lds - 1024 : 0;
 }
 
-// Launches the PSD projection of one order group (the kernel instance that
-// matches the group's mode; dynamic LDS above the default limit is requested
-// once per instance).
-template <class X> void psd_launch(const PsdGroup &g, X *v, const int *gate, hipStream_t st) {
-    static bool raised[3] = {false, false, false};
-    auto go = [&](auto kernel, int mode) {
-        if (!raised[mode] && g.lds > 0) {
-            HIP_OK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)psd_lds_budget()));
-            raised[mode] = true;
-        }
-        hipLaunchKernelGGL(kernel, dim3(g.count), dim3(kPsdThreads), g.lds, st, v, g.starts, g.order, g.work, gate);
-    };
-    if (g.mode == 0) go(psd_project_kernel<X, 0>, 0);
-    else if (g.mode == 1) go(psd_project_kernel<X, 1>, 1);
-    else go(psd_project_kernel<X, 2>, 2);
+// Launches the PSD projection of every block; `site` selects which kept
+// eigenvector set warm-starts it (0: the iterate, 1: the dual residual).
+// Dynamic LDS above the default limit is requested once per instance.
+template <class X> void psd_launch(const PsdSet &g, X *v, int site, const int *gate, hipStream_t st) {
+    static bool raised = false;
+    if (!g.count) return;
+    if (!raised && g.lds > 0) {
+        HIP_OK(hipFuncSetAttribute((const void *)psd_project_kernel<X>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)psd_lds_budget()));
+        raised = true;
+    }
+    hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kPsdThreads), g.lds, st, v, (const PsdBlock *)g.blocks,
+                       g.work, site, gate);
 }
 
 class DevConeSet {
@@ -166,7 +164,7 @@ class DevConeSet {
     double *lsec_scratch = nullptr;              // 4 (v_dim + 2) values per long section block
     long long *lsec_off = nullptr;
     char *fixed = nullptr;                       // per-slot fixed flags, when sections exist
-    std::vector<PsdGroup> groups;
+    PsdSet pset;                                 // every PSD block
     char *mask = nullptr;
     int total = 0;                               // all blocks, in input order:
     int *all_start = nullptr, *all_len = nullptr;  // first slot and slot count
@@ -274,25 +272,36 @@ class DevConeSet {
             std::vector<char> hf(b->is_fixed, b->is_fixed + n);
             fixed = up<char>(hf, hf.size());
         }
-        std::sort(ps.begin(), ps.end());
-        for (size_t i = 0; i < ps.size();) {
-            size_t j = i;
-            std::vector<int> starts;
-            while (j < ps.size() && ps[j].first == ps[i].first) starts.push_back(ps[j++].second);
-            PsdGroup g;
-            g.order = ps[i].first;
-            g.count = (int)starts.size();
-            g.starts = up<int>(starts, starts.size());
+        // PSD blocks, largest orders first (stable in input order otherwise)
+        std::stable_sort(ps.begin(), ps.end(),
+                         [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first > y.first; });
+        if (!ps.empty()) {
             const size_t budget = psd_lds_budget();
-            g.mode = psd_lds_bytes(g.order, 0) <= budget ? 0 : (psd_lds_bytes(g.order, 1) <= budget ? 1 : 2);
             // PDHCG_PSD_MODE=1|2 moves V' (1) or A and V' (2) to the work buffer
             // on orders that would fit LDS (the tests run every mode)
-            if (const char *fm = std::getenv("PDHCG_PSD_MODE")) g.mode = std::max(g.mode, std::min(2, std::atoi(fm)));
-            g.lds = psd_lds_bytes(g.order, g.mode);
-            const size_t wd = psd_work_doubles(g.order, g.mode) * (size_t)g.count;
-            if (wd) g.work = raw<double>(wd);
-            groups.push_back(g);
-            i = j;
+            const char *fm = std::getenv("PDHCG_PSD_MODE");
+            std::vector<PsdBlock> hb(ps.size());
+            long long work = 0;
+            for (size_t i = 0; i < ps.size(); ++i) {
+                PsdBlock &pb = hb[i];
+                pb.order = ps[i].first;
+                pb.start = ps[i].second;
+                pb.mode = psd_lds_bytes(pb.order, 0) <= budget ? 0 : (psd_lds_bytes(pb.order, 1) <= budget ? 1 : 2);
+                if (fm) pb.mode = std::max(pb.mode, std::min(2, std::atoi(fm)));
+                pb.pad = 0;
+                pb.work = work;
+                work += (long long)psd_work_doubles(pb.order, pb.mode);
+                pset.lds = std::max(pset.lds, psd_lds_bytes(pb.order, pb.mode));
+            }
+            pset.count = (int)hb.size();
+            pset.blocks = up<PsdBlock>(hb, hb.size());
+            pset.work = raw<double>((size_t)work);
+            if (!measuring()) {
+                hipLaunchKernelGGL(psd_identity_kernel, dim3(pset.count), dim3(kBlock), 0, nullptr,
+                                   (const PsdBlock *)pset.blocks, pset.work);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipStreamSynchronize(nullptr));
+            }
         }
         mask = up<char>(hmask, hmask.size());
         total = (int)hs.size();
@@ -330,7 +339,7 @@ class DevConeSet {
         if (lsec.count)
             hipLaunchKernelGGL(section_long_project_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, v, d,
                                (const X *)nullptr, 0.0, fixed, lsec, lsec_scratch, lsec_off, gate);
-        for (const PsdGroup &g : groups) psd_launch<X>(g, v, gate, st);
+        psd_launch<X>(pset, v, 0, gate, st);
     }
 
     // Cone step of the diagonal-Q prox: projection onto diag(d) K in the
@@ -396,7 +405,7 @@ class DevConeSet {
         if (psd) {
             hipLaunchKernelGGL(psd_rc_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, scratch, dres, c, qx, aty,
                                mask, n);
-            for (const PsdGroup &g : groups) psd_launch<X>(g, scratch, nullptr, st);
+            psd_launch<X>(pset, scratch, 1, nullptr, st);
             hipLaunchKernelGGL(psd_dres_kernel<X>, dim3(blocks_for(n)), dim3(kBlock), 0, st, dres, scratch, d, mask,
                                n);
         }
@@ -422,11 +431,9 @@ class DevConeSet {
         lsec_scratch = nullptr;
         lsec_off = nullptr;
         fixed = nullptr;
-        for (PsdGroup &g : groups) {
-            if (g.starts) (void)hipFree(g.starts);
-            if (g.work) (void)hipFree(g.work);
-        }
-        groups.clear();
+        if (pset.blocks) (void)hipFree(pset.blocks);
+        if (pset.work) (void)hipFree(pset.work);
+        pset = PsdSet();
         for (int *p : {all_start, all_len})
             if (p) (void)hipFree(p);
         if (mask) (void)hipFree(mask);
@@ -447,7 +454,7 @@ class DevConeSet {
         lsec_scratch = nullptr;
         lsec_off = nullptr;
         fixed = mask = nullptr;
-        groups.clear();
+        pset = PsdSet();
         total = 0;
         all_start = all_len = nullptr;
         arena_ = nullptr;
@@ -470,6 +477,7 @@ class DevConeSet {
         return p;
     }
     template <class T> T *raw(size_t count) { return arena_ ? arena_->take<T>(count) : dalloc<T>(count); }
+    bool measuring() const { return arena_ && arena_->measuring(); }
 };
 
 }  // namespace
@@ -2112,38 +2120,61 @@ void pdhcg_solver_free(pdhcg_solver_t *s) {
 
 // Standalone cone operations on host vectors (one device round trip each).
 namespace {
+// Device buffers and the stream of one standalone cone call, released on
+// every exit (hipFree waits for the device, so kernels still queued after
+// an error finish first).
+struct CallBuffers {
+    hipStream_t st = nullptr;
+    std::vector<void *> held;
+    double *keep(double *p) {
+        held.push_back(p);
+        return p;
+    }
+    ~CallBuffers() {
+        for (void *p : held)
+            if (p) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+// count > 1 (projections only): `count` vectors of n slots, one after
+// another in in/out, projected in order through one cone set.
 int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, const double *in, double *out, int n,
-               bool dual) {
-    if (!cones || !in || !out || n < 0 || cones->num_cones < 0 || (cones->num_cones > 0 && !cones->start_idx)) {
+               bool dual, int count = 1) {
+    if (!cones || !in || !out || n < 0 || count < 1 || (dual && count != 1) || cones->num_cones < 0 ||
+        (cones->num_cones > 0 && !cones->start_idx)) {
         std::fprintf(stderr, "[interface] %s: invalid arguments.\n", name);
         return -1;
     }
     try {
-        hipStream_t st;
-        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        CallBuffers cb;
+        HIP_OK(hipStreamCreateWithFlags(&cb.st, hipStreamNonBlocking));
+        hipStream_t st = cb.st;
         DevConeSet set;
         set.build(cones, n);
         if (dual && (set.sec.count || set.lsec.count))
             throw std::runtime_error("blocks with fixed slots have no dual-cone residual without a primal point");
         std::vector<double> dh((size_t)n, 1.0);
         if (d) std::copy(d, d + n, dh.begin());
-        std::vector<double> vh(in, in + n);
-        double *dd = dupload<double>(dh, n), *v = dupload<double>(vh, n), *res = nullptr, *psd_buf = nullptr,
-               *zero = dzeros<double>(n);
+        double *dd = cb.keep(dupload<double>(dh, n));
+        double *v = cb.keep(dalloc<double>((size_t)n));
+        double *zero = cb.keep(dzeros<double>(n));
         if (dual) {
-            res = dupload<double>(std::vector<double>(out, out + n), n);
-            psd_buf = dzeros<double>(n);
+            HIP_OK(hipMemcpy(v, in, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+            double *res = cb.keep(dupload<double>(std::vector<double>(out, out + n), n));
+            double *psd_buf = cb.keep(dzeros<double>(n));
             set.dual_residual<double>(res, psd_buf, zero, v, nullptr, zero, dd, n, st);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(out, res, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         } else {
-            set.project<double>(v, dd, false, nullptr, st);
+            for (int i = 0; i < count; ++i) {
+                HIP_OK(hipMemcpyAsync(v, in + (size_t)i * n, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+                set.project<double>(v, dd, false, nullptr, st);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipMemcpyAsync(out + (size_t)i * n, v, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+            }
         }
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(out, dual ? res : v, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        for (void *p : {(void *)dd, (void *)v, (void *)res, (void *)psd_buf, (void *)zero})
-            if (p) (void)hipFree(p);
-        set.release();
-        (void)hipStreamDestroy(st);
         return 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[interface] %s: %s\n", name, e.what());
@@ -2154,6 +2185,10 @@ int cone_entry(const char *name, const cone_blocks_t *cones, const double *d, co
 
 int pdhcg_project_cones(const cone_blocks_t *cones, const double *d, double *x, int n) {
     return cone_entry("pdhcg_project_cones", cones, d, x, x, n, false);
+}
+
+int pdhcg_project_cones_seq(const cone_blocks_t *cones, const double *d, double *x, int n, int count) {
+    return cone_entry("pdhcg_project_cones_seq", cones, d, x, x, n, false, count);
 }
 
 // ---- allocation-free variant of pdhcg_project_cones ----

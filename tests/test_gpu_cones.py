@@ -121,6 +121,58 @@ def test_psd_large_orders_match_oracle(mode, monkeypatch):
         assert np.max(np.abs(got[sl] - exp)) <= 1e-12 * scale, (k, np.max(np.abs(got[sl] - exp)))
 
 
+def _psd_block_set(rng, orders):
+    blocks = [(k, _mat_to_svec(psd_test_matrix(k, rng, ("random", "clustered", "rank1")[i % 3])))
+              for i, k in enumerate(orders)]
+    specs, pos = [], 0
+    for k, x in blocks:
+        specs.append((PSD, pos, k))
+        pos += x.size
+    return blocks, specs, pos
+
+
+def _check_psd(got, blocks, specs, xs):
+    for (k, _), sp_ in zip(blocks, specs):
+        L = k * (k + 1) // 2
+        sl = slice(sp_[1], sp_[1] + L)
+        exp = orc_project(PSD, xs[sl].copy(), k)
+        scale = 1.0 + np.max(np.abs(xs[sl]))
+        assert np.all(np.isfinite(got[sl]))
+        assert np.max(np.abs(got[sl] - exp)) <= 1e-12 * scale, (k, np.max(np.abs(got[sl] - exp)))
+
+
+def test_psd_mixed_orders_one_launch():
+    # every order in one launch, largest first: orders 1..3, odd orders
+    # (padded), the LDS limits of both layouts (A and V' in LDS up to about
+    # 100, A alone up to about 140) and order 150, where both live in the
+    # global work buffer
+    rng = np.random.default_rng(91)
+    blocks, specs, n = _psd_block_set(rng, [1, 2, 3, 16, 17, 64, 99, 100, 101, 128, 141, 142, 150, 7, 128])
+    x = np.concatenate([b for _, b in blocks])
+    got = pdhcg_amd.project_cones(specs, x, np.ones(n))
+    _check_psd(got, blocks, specs, x)
+
+
+@pytest.mark.parametrize("mode", ["", "1", "2"])
+def test_psd_warm_sequence_matches_oracle(mode, monkeypatch):
+    # one cone set projects a sequence: a point, two perturbations of it
+    # (warm starts close to the answer), an unrelated point (a warm start far
+    # from it) and the first point again; each result is that point's
+    # projection to 1e-12
+    if mode:
+        monkeypatch.setenv("PDHCG_PSD_MODE", mode)
+    rng = np.random.default_rng(92 + len(mode))
+    blocks, specs, n = _psd_block_set(rng, [16, 33, 64, 97, 128])
+    x0 = np.concatenate([b for _, b in blocks])
+    seq = [x0, x0 + 1e-3 * rng.standard_normal(n), x0 + 1e-2 * rng.standard_normal(n),
+           np.concatenate([_mat_to_svec(psd_test_matrix(k, rng, "random")) for k, _ in blocks]), x0]
+    got = pdhcg_amd.project_cones_seq(specs, np.stack(seq), np.ones(n))
+    for i, xs in enumerate(seq):
+        _check_psd(got[i], blocks, specs, xs)
+    # the repeated point: the same projection to rounding whatever came before
+    assert np.max(np.abs(got[4] - got[0])) <= 1e-12 * (1.0 + np.max(np.abs(x0)))
+
+
 @pytest.mark.parametrize("t", [SOC, RSOC, POW, PSD])
 def test_dual_residual_matches_oracle(t):
     rng = np.random.default_rng(200 + t)
