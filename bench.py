@@ -295,6 +295,13 @@ def run_conic(seconds, eps, cpu_seconds, with_cpu):
     return out
 
 
+def run_warm_large(args):
+    """The warm start on the family at the stated instance size: n = 1e5,
+    m = 5e4, nnz(A) = 1e6, rank 50 (network 16 -> 1024 -> 1024 -> 150000)."""
+    return run_warm(args.warm_large_train, args.warm_large_test, args.eps, n=100_000, m=50_000, nnz=1_000_000,
+                    singles=2, epochs=200, tols=(args.eps,))
+
+
 def capi_const(name):
     from pdhcg_amd import capi
     return getattr(capi, name)
@@ -308,26 +315,31 @@ def mlp_step_flops(in_dim, hidden, out_dim, batch):
     return 2.0 * batch * (2 * i * h + 3 * h * h + 3 * h * o)
 
 
-def run_warm(train_count, test_count, eps):
+def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4, epochs=300, tols=None):
     """The learned warm start on the parametric sparse QP + low-rank family
-    (n = 2000, m = 1000, nnz = 20000, rank 50, z in R^16): training solutions
-    from the batched solver at 1e-4, the 16 -> 1024 -> 1024 -> (n + m) network
-    trained on the device, then unseen instances solved cold and warm."""
+    (n x m with nnz(A) nonzeros, rank 50, z in R^16; defaults n = 2000,
+    m = 1000, nnz = 20000): training solutions from the batched solver at
+    1e-4, the 16 -> 1024 -> 1024 -> (n + m) network trained on the device,
+    then unseen instances solved cold, warm, primal-only warm and warm with
+    the guard (predicted start kept only where its KKT error at iteration 0
+    is below the cold start's; the guard's own evaluation is in its time)."""
     import numpy as np
     import pdhcg_amd
     from pdhcg_amd import warmstart, workloads
-    fam = workloads.ParametricFamily()
+    fam = workloads.ParametricFamily(n=n, m=m, nnz=nnz)
     Ztr, ptr = fam.sample(train_count, 11)
     Zte, pte = fam.sample(test_count, 12)
+    log(f"[bench] warm n={fam.n}: {train_count} training and {test_count} test instances generated")
     t = time.perf_counter()
-    sols = pdhcg_amd.solve_batch(ptr, eps=1e-4, iteration_limit=100000, time_limit=300.0)
+    # verbose 1: the batch reports each window (progress for long batches)
+    sols = pdhcg_amd.solve_batch(ptr, eps=1e-4, iteration_limit=100000, time_limit=300.0, verbose=1)
     gen_s = time.perf_counter() - t
     ok = [i for i, s_ in enumerate(sols) if s_ is not None and s_.status == "OPTIMAL"]
     X = np.stack([sols[i].x for i in ok])
     Y = np.stack([sols[i].y for i in ok])
     del sols, ptr
     log(f"[bench] warm: {len(ok)} of {train_count} training solutions in {gen_s:.1f}s")
-    epochs, batch = 300, 256
+    batch = 256
     ws = warmstart.WarmStart(fam.z_dim, fam.n, fam.m, hidden=1024, seed=0)
     t = time.perf_counter()
     loss = ws.fit(Ztr[ok], X, Y, epochs=epochs, batch=batch, lr=1e-3)
@@ -345,7 +357,7 @@ def run_warm(train_count, test_count, eps):
            "train_instances": len(ok), "test_instances": test_count, "train_solve_s": gen_s,
            "train_s": train_s, "train_steps": steps, "train_loss": loss, "train_tflops": tflops,
            "train_mfma_frac": tflops / 2500.0, "predict_s": predict_s}
-    for tol in (eps, 1e-4):
+    for tol in (tols or (eps, 1e-4)):
         t = time.perf_counter()
         cold = pdhcg_amd.solve_batch(pte, eps=tol, iteration_limit=100000, time_limit=300.0)
         cold_s = time.perf_counter() - t
@@ -355,6 +367,11 @@ def run_warm(train_count, test_count, eps):
         t = time.perf_counter()
         primal = pdhcg_amd.solve_batch(primal_p, eps=tol, iteration_limit=100000, time_limit=300.0)
         primal_s = time.perf_counter() - t
+        t = time.perf_counter()
+        guard_p, kept = ws.guarded(pte, Zte, eps=tol)
+        guarded = pdhcg_amd.solve_batch(guard_p, eps=tol, iteration_limit=100000, time_limit=300.0)
+        guarded_s = time.perf_counter() - t
+        ig = np.array([s_.iterations for s_ in guarded])
         ip = np.array([s_.iterations for s_ in primal])
         ic = np.array([s_.iterations for s_ in cold])
         iw = np.array([s_.iterations for s_ in warm])
@@ -368,6 +385,11 @@ def run_warm(train_count, test_count, eps):
                "primal_only_batch_s": primal_s, "primal_only_iterations_mean": float(ip.mean()),
                "primal_only_iterations_max": int(ip.max()),
                "primal_only_optimal": [s_.status for s_ in primal].count("OPTIMAL"),
+               # the guard: predicted start kept where its iteration-0 KKT
+               # error beats the cold start's (guard evaluation included)
+               "guarded_batch_s": guarded_s, "guarded_kept": int(kept.sum()),
+               "guarded_iterations_mean": float(ig.mean()), "guarded_iterations_max": int(ig.max()),
+               "guarded_optimal": [s_.status for s_ in guarded].count("OPTIMAL"),
                "optimal": [s_.status for s_ in warm].count("OPTIMAL"), "max_rel_objective_diff": obj}
         if tol == eps:
             xs = np.stack([s_.x for s_ in cold])
@@ -376,7 +398,7 @@ def run_warm(train_count, test_count, eps):
             yp = np.stack([p_.dual_start for p_ in warm_p])
             out["prediction_rel_error"] = {"x": float(np.linalg.norm(xp - xs) / np.linalg.norm(xs)),
                                            "y": float(np.linalg.norm(yp - ys) / np.linalg.norm(ys))}
-            sample = list(range(min(4, test_count)))
+            sample = list(range(min(singles, test_count)))
             t = time.perf_counter()
             for k in sample:
                 pdhcg_amd.solve(pte[k], eps=tol, iteration_limit=100000)
@@ -385,15 +407,22 @@ def run_warm(train_count, test_count, eps):
             for k in sample:
                 pdhcg_amd.solve(warm_p[k], eps=tol, iteration_limit=100000)
             single_warm = (time.perf_counter() - t) / len(sample)
+            t = time.perf_counter()
+            for k in sample:
+                gp, _ = ws.guarded([pte[k]], Zte[k:k + 1], batch=False, eps=tol)
+                pdhcg_amd.solve(gp[0], eps=tol, iteration_limit=100000)
+            single_guarded = (time.perf_counter() - t) / len(sample)
             rec["single_device_cold_s"] = single_cold
             rec["single_device_warm_s"] = single_warm
+            rec["single_device_guarded_s"] = single_guarded
             # the plain form (one cold FP64 solve per problem) against the
             # batched warm-started form, per problem, same tolerance
             rec["speedup_batched_warm_vs_single_cold"] = single_cold / (warm_s / test_count)
             rec["speedup_batched_primal_only_vs_single_cold"] = single_cold / (primal_s / test_count)
         out["tol_%g" % tol] = rec
-        log(f"[bench] warm: tol {tol:g}: iterations cold {ic.mean():.0f} warm {iw.mean():.0f} "
-            f"primal-only {ip.mean():.0f} (max {ic.max()}/{iw.max()}/{ip.max()})")
+        log(f"[bench] warm n={fam.n}: tol {tol:g}: batch s cold {cold_s:.2f} warm {warm_s:.2f} primal-only "
+            f"{primal_s:.2f} guarded {guarded_s:.2f}; iterations cold {ic.mean():.0f} warm {iw.mean():.0f} "
+            f"primal-only {ip.mean():.0f} guarded {ig.mean():.0f} (max {ic.max()}/{iw.max()}/{ip.max()}/{ig.max()})")
     ws.close()
     return out
 
@@ -421,6 +450,9 @@ def main():
                     help="time limit of each conic-mix solve (0: skip the conic mix)")
     ap.add_argument("--conic-only", action="store_true",
                     help="only run the conic-mix section (prints {\"conic_mix\": ...})")
+    ap.add_argument("--warm-large-train", type=int, default=0, metavar="N",
+                    help="training instances of the n=1e5, m=5e4, nnz=1e6 parametric family (0: skip that section)")
+    ap.add_argument("--warm-large-test", type=int, default=64, metavar="N")
     ap.add_argument("--warm-only", action="store_true",
                     help="only run the learned warm start section (prints {\"warm_start\": ...})")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -435,7 +467,12 @@ def main():
         sys.exit(2)
 
     if args.warm_only:
-        print(json.dumps({"warm_start": run_warm(args.warm_train, args.warm_test, args.eps)}), flush=True)
+        out = {}
+        if args.warm_train > 0:
+            out["warm_start"] = run_warm(args.warm_train, args.warm_test, args.eps)
+        if args.warm_large_train > 0:
+            out["warm_start_large"] = run_warm_large(args)
+        print(json.dumps(out), flush=True)
         return
     if args.conic_only:
         print(json.dumps({"conic_mix": run_conic(args.conic_seconds, args.eps, args.cpu_seconds, not args.no_cpu)}),
@@ -551,6 +588,8 @@ def main():
         out["conic_mix"] = run_conic(args.conic_seconds, args.eps, args.cpu_seconds, not args.no_cpu)
     if args.warm_train > 0:
         out["warm_start"] = run_warm(args.warm_train, args.warm_test, args.eps)
+    if args.warm_large_train > 0:
+        out["warm_start_large"] = run_warm_large(args)
     print(json.dumps(out), flush=True)
 
 

@@ -345,6 +345,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         auto vpool = raw_buffer<double>((size_t)voff[B]);
         auto cones = raw_buffer<Cone>((size_t)coff[B]);
         auto slabs = raw_buffer<double>((size_t)soff[B]);
+        std::atomic<bool> pack_error{false};
         parallel_for(B, [&](int b) {
             Packed &q = pk[b];
             Inst I = q.inst;
@@ -364,6 +365,27 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             std::copy(q.av.begin(), q.av.end(), vpool.get() + vo);
             std::copy(q.cones.begin(), q.cones.end(), cones.get() + coff[b]);
             std::copy(q.slab.begin(), q.slab.end(), slabs.get() + soff[b]);
+            // every offset the kernel derives from I stays inside this
+            // problem's share of the pools (a failure is a packing bug)
+            const long long nnz = (long long)q.inst.at_rp - q.inst.a_ci;
+            const int *rp = ipool.get() + I.a_rp, *trp = ipool.get() + I.at_rp;
+            bool ok = I.a_ci == I.a_rp + I.m + 1 && (long long)I.at_rp == I.a_ci + nnz &&
+                      I.at_ci == I.at_rp + I.n + 1 && I.at_ci + nnz <= ioff[b + 1] && rp[0] == 0 &&
+                      rp[I.m] == nnz && trp[0] == 0 && trp[I.n] == nnz && I.at_v == I.a_v + nnz &&
+                      I.rdense == I.at_v + nnz && I.dmat >= I.rdense && I.dmat <= voff[b + 1] &&
+                      I.vec + slab_size(I.n, I.m, I.k, I.ncones) <= soff[b + 1] && I.cone0 + I.ncones <= coff[b + 1];
+            for (long long e = 0; ok && e < nnz; ++e) {
+                const int c = ipool[(size_t)I.a_ci + e], r = ipool[(size_t)I.at_ci + e];
+                ok = c >= 0 && c < I.n && r >= 0 && r < I.m;
+            }
+            for (int c = 0; ok && c < I.ncones; ++c) {
+                const Cone &cn = cones[(size_t)I.cone0 + c];
+                const int len = cn.type == CONE_EXPONENTIAL || cn.type == CONE_POWER ? 3 : cn.vdim + 2;
+                ok = cn.start >= 0 && (long long)cn.start + len <= I.m &&
+                     (c >= I.ncones - I.nlong || cn.type == CONE_EXPONENTIAL || cn.type == CONE_POWER ||
+                      cn.vdim <= kMaxThreadCone);
+            }
+            if (!ok) pack_error.store(true);
             insts[b] = I;
             states[b] = q.st;
             // the pools hold the data now
@@ -371,6 +393,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             std::vector<double>().swap(q.av);
             std::vector<double>().swap(q.slab);
         });
+        if (pack_error.load()) fail("packing produced an offset outside a problem's pools (internal error)");
         Params dp;
         dp.ef = std::max(1, P.termination_evaluation_frequency);
         dp.iteration_limit = P.termination_criteria.iteration_limit;
@@ -419,14 +442,20 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         std::vector<Out> out((size_t)B);
         bool time_up = false, cancelled = false;
         for (;;) {
+            const auto tw = std::chrono::steady_clock::now();
             hipLaunchKernelGGL(batch_solve_kernel, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool, d_vpool,
                                d_slabs, d_cones, d_states, d_out, window);
             ++windows;
             HIP_OK(hipGetLastError());
             HIP_OK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(Out), hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
-            bool all = true;
-            for (const Out &o : out) all = all && o.status != TERMINATION_REASON_UNSPECIFIED;
+            int running = 0;
+            for (const Out &o : out) running += o.status == TERMINATION_REASON_UNSPECIFIED ? 1 : 0;
+            if (P.verbose >= 1)  // the tail: how many problems each window leaves running
+                std::fprintf(stderr, "[batch] window %d (%d passes): %.3fs, %d of %d problems still running\n", windows,
+                             window, std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count(),
+                             running, B);
+            const bool all = running == 0;
             if (all) break;
             if (g_pdhcg_cancel) {  // user interrupt, checked between windows
                 cancelled = true;

@@ -808,6 +808,8 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
 
 constexpr int kPsdMaxSweeps = 60;
 constexpr int kPsdThreads = 512;
+constexpr int kPsdChunk = 4;    // 2x2 block updates a thread loads before it stores (phase 2)
+constexpr int kPsdChunkV = 8;   // V' row-pair updates a thread loads before it stores
 
 // One PSD block of a launch.
 struct PsdBlock {
@@ -1009,44 +1011,78 @@ __global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const Ps
                 pq[2 * i + 1] = q;
             }
             __syncthreads();
-            // phase 2: every 2x2 block by its owner, then the rows of V'
-            for (int w = tid; w < np * np; w += nt) {
-                const int P = w / np, R = w - P * np;
-                const int p = pq[2 * P], q = pq[2 * P + 1], r = pq[2 * R], s = pq[2 * R + 1];
-                double *Ap = A + (size_t)p * kp, *Aq = A + (size_t)q * kp;
-                const double e00 = Ap[r], e01 = Ap[s], e10 = Aq[r], e11 = Aq[s];
-                if (P == R) {
-                    const double c1 = cs[2 * P], s1 = cs[2 * P + 1];
-                    const double m00 = c1 * e00 - s1 * e10, m01 = c1 * e01 - s1 * e11;
-                    const double m10 = s1 * e00 + c1 * e10, m11 = s1 * e01 + c1 * e11;
-                    Ap[p] = c1 * m00 - s1 * m01;
-                    Aq[q] = s1 * m10 + c1 * m11;
-                    Ap[q] = 0.0;
-                    Aq[p] = 0.0;
-                    continue;
+            // phase 2: every 2x2 block by its owner, then the rows of V'.
+            // Each thread loads kPsdChunk items before it stores any, so
+            // their LDS (or L2) round trips overlap instead of queueing
+            // behind the stores of the item before.
+            for (int w0 = tid; w0 < np * np; w0 += kPsdChunk * nt) {
+                int off[kPsdChunk][4];
+                double e[kPsdChunk][4], rot[kPsdChunk][4];
+                bool tr[kPsdChunk], diag[kPsdChunk], live[kPsdChunk];
+#pragma unroll
+                for (int j = 0; j < kPsdChunk; ++j) {
+                    const int w = w0 + j * nt;
+                    live[j] = w < np * np;
+                    const int P = live[j] ? w / np : 0, R = live[j] ? w - P * np : 0;
+                    const int p = pq[2 * P], q = pq[2 * P + 1], r = pq[2 * R], s2 = pq[2 * R + 1];
+                    off[j][0] = p * kp + r;
+                    off[j][1] = p * kp + s2;
+                    off[j][2] = q * kp + r;
+                    off[j][3] = q * kp + s2;
+                    // orientation (lo, hi): lo = min(P, R); b is the (lo, hi) block
+                    tr[j] = P > R;
+                    diag[j] = P == R;
+                    const int lo = tr[j] ? R : P, hi = tr[j] ? P : R;
+                    rot[j][0] = cs[2 * lo];
+                    rot[j][1] = cs[2 * lo + 1];
+                    rot[j][2] = cs[2 * hi];
+                    rot[j][3] = cs[2 * hi + 1];
+                    for (int u = 0; u < 4; ++u) e[j][u] = live[j] ? A[off[j][u]] : 0.0;
                 }
-                // orientation (lo, hi): lo = min(P, R); b is the (lo, hi) block
-                const bool tr = P > R;
-                const int lo = tr ? R : P, hi = tr ? P : R;
-                const double c1 = cs[2 * lo], s1 = cs[2 * lo + 1], c2 = cs[2 * hi], s2 = cs[2 * hi + 1];
-                const double b00 = e00, b01 = tr ? e10 : e01, b10 = tr ? e01 : e10, b11 = e11;
-                const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
-                const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
-                const double n00 = c2 * m00 - s2 * m01, n01 = s2 * m00 + c2 * m01;
-                const double n10 = c2 * m10 - s2 * m11, n11 = s2 * m10 + c2 * m11;
-                Ap[r] = n00;
-                Ap[s] = tr ? n10 : n01;
-                Aq[r] = tr ? n01 : n10;
-                Aq[s] = n11;
+#pragma unroll
+                for (int j = 0; j < kPsdChunk; ++j) {
+                    if (!live[j]) continue;
+                    const double c1 = rot[j][0], s1 = rot[j][1], c2 = rot[j][2], s2 = rot[j][3];
+                    const double b00 = e[j][0], b01 = tr[j] ? e[j][2] : e[j][1], b10 = tr[j] ? e[j][1] : e[j][2],
+                                 b11 = e[j][3];
+                    const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
+                    const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
+                    if (diag[j]) {
+                        A[off[j][0]] = c1 * m00 - s1 * m01;
+                        A[off[j][3]] = s1 * m10 + c1 * m11;
+                        A[off[j][1]] = 0.0;
+                        A[off[j][2]] = 0.0;
+                        continue;
+                    }
+                    const double n00 = c2 * m00 - s2 * m01, n01 = s2 * m00 + c2 * m01;
+                    const double n10 = c2 * m10 - s2 * m11, n11 = s2 * m10 + c2 * m11;
+                    A[off[j][0]] = n00;
+                    A[off[j][1]] = tr[j] ? n10 : n01;
+                    A[off[j][2]] = tr[j] ? n01 : n10;
+                    A[off[j][3]] = n11;
+                }
             }
-            for (int w = tid; w < np * kp; w += nt) {
-                const int P = w / kp, i = w - P * kp;
-                const double c = cs[2 * P], s = cs[2 * P + 1];
-                if (s == 0.0) continue;
-                double *vp = Vt + (size_t)pq[2 * P] * kp, *vq = Vt + (size_t)pq[2 * P + 1] * kp;
-                const double a = vp[i], b = vq[i];
-                vp[i] = c * a - s * b;
-                vq[i] = s * a + c * b;
+            for (int w0 = tid; w0 < np * kp; w0 += kPsdChunkV * nt) {
+                int op[kPsdChunkV], oq[kPsdChunkV];
+                double va[kPsdChunkV], vb[kPsdChunkV], cc[kPsdChunkV], ss[kPsdChunkV];
+#pragma unroll
+                for (int j = 0; j < kPsdChunkV; ++j) {
+                    const int w = w0 + j * nt;
+                    const bool live = w < np * kp;
+                    const int P = live ? w / kp : 0, i = live ? w - P * kp : 0;
+                    cc[j] = cs[2 * P];
+                    ss[j] = live ? cs[2 * P + 1] : 0.0;
+                    op[j] = pq[2 * P] * kp + i;
+                    oq[j] = pq[2 * P + 1] * kp + i;
+                    va[j] = ss[j] != 0.0 ? Vt[op[j]] : 0.0;
+                    vb[j] = ss[j] != 0.0 ? Vt[oq[j]] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < kPsdChunkV; ++j) {
+                    if (ss[j] == 0.0) continue;
+                    Vt[op[j]] = cc[j] * va[j] - ss[j] * vb[j];
+                    Vt[oq[j]] = ss[j] * va[j] + cc[j] * vb[j];
+                }
             }
             __syncthreads();
         }

@@ -123,6 +123,7 @@ void merge_cone_scaling(std::vector<double> &s, const cone_blocks_t *b, Phase ph
 struct Scaler {
     ScaledProblem &w;
     const cone_blocks_t *cones, *acones;
+    MatrixPasses *dev;  // entry-wise passes of Ruiz, Pock-Chambolle and bound/objective (null: here)
 
     // Divides the problem by row factors cr and column factors vr and
     // accumulates them (reference :256-309 and the cum_* updates).
@@ -137,6 +138,12 @@ struct Scaler {
             w.cu[i] /= cr[i];
             w.g[i] /= cr[i];
         }
+        for (int j = 0; j < w.n; ++j) w.var_rescale[j] *= vr[j];
+        for (int i = 0; i < w.m; ++i) w.con_rescale[i] *= cr[i];
+        if (dev) {
+            dev->divide(cr, vr);
+            return;
+        }
         for (int r = 0; r < w.m; ++r)
             for (int e = w.A.rp[r]; e < w.A.rp[(size_t)r + 1]; ++e) w.A.v[e] /= (cr[r] * vr[w.A.ci[e]]);
         for (int r = 0; r < w.n; ++r)
@@ -144,8 +151,6 @@ struct Scaler {
         if (w.R.nnz() > 0)
             for (int r = 0; r < w.k; ++r)
                 for (int e = w.R.rp[r]; e < w.R.rp[(size_t)r + 1]; ++e) w.R.v[e] /= vr[w.R.ci[e]];
-        for (int j = 0; j < w.n; ++j) w.var_rescale[j] *= vr[j];
-        for (int i = 0; i < w.m; ++i) w.con_rescale[i] *= cr[i];
     }
 
     static double root_or_one(double v) { return v < kScaleEps ? 1.0 : std::sqrt(v); }
@@ -155,13 +160,15 @@ struct Scaler {
         for (int it = 0; it < iters; ++it) {
             std::fill(cr.begin(), cr.end(), 0.0);
             std::fill(vr.begin(), vr.end(), 0.0);
-            for (int r = 0; r < w.m; ++r)
-                for (int e = w.A.rp[r]; e < w.A.rp[(size_t)r + 1]; ++e) {
-                    const double a = std::fabs(w.A.v[e]);
-                    const int c = w.A.ci[e];
-                    if (a > vr[c]) vr[c] = a;
-                    if (a > cr[r]) cr[r] = a;
-                }
+            if (dev) dev->abs_max(cr, vr);
+            else
+                for (int r = 0; r < w.m; ++r)
+                    for (int e = w.A.rp[r]; e < w.A.rp[(size_t)r + 1]; ++e) {
+                        const double a = std::fabs(w.A.v[e]);
+                        const int c = w.A.ci[e];
+                        if (a > vr[c]) vr[c] = a;
+                        if (a > cr[r]) cr[r] = a;
+                    }
             for (double &v : vr) v = root_or_one(v);
             for (double &v : cr) v = root_or_one(v);
             merge_cone_scaling(vr, cones, Phase::Ruiz, preserve);
@@ -172,12 +179,14 @@ struct Scaler {
 
     void pock_chambolle(double alpha, bool preserve) {
         std::vector<double> cr((size_t)w.m, 0.0), vr((size_t)w.n, 0.0);
-        for (int r = 0; r < w.m; ++r)
-            for (int e = w.A.rp[r]; e < w.A.rp[(size_t)r + 1]; ++e) {
-                const double a = std::fabs(w.A.v[e]);
-                vr[w.A.ci[e]] += std::pow(a, 2.0 - alpha);
-                cr[r] += std::pow(a, alpha);
-            }
+        if (dev) dev->pow_sums(alpha, cr, vr);
+        else
+            for (int r = 0; r < w.m; ++r)
+                for (int e = w.A.rp[r]; e < w.A.rp[(size_t)r + 1]; ++e) {
+                    const double a = std::fabs(w.A.v[e]);
+                    vr[w.A.ci[e]] += std::pow(a, 2.0 - alpha);
+                    cr[r] += std::pow(a, alpha);
+                }
         for (double &v : vr) v = root_or_one(v);
         for (double &v : cr) v = root_or_one(v);
         merge_cone_scaling(vr, cones, Phase::PockChambolle, preserve);
@@ -271,6 +280,10 @@ struct Scaler {
             w.ub[j] *= w.cbr;
             w.c[j] *= w.ovr;
         }
+        if (dev) {
+            dev->multiply_qr(w.ovr / w.cbr, std::sqrt(w.ovr / w.cbr));
+            return;
+        }
         for (double &v : w.Q.v) v *= w.ovr / w.cbr;
         if (w.R.nnz() > 0) {
             const double f = std::sqrt(w.ovr / w.cbr);
@@ -291,7 +304,7 @@ quad_obj_type_t classify_q(const ScaledProblem &w) {
 
 }  // namespace
 
-ScaledProblem scale_problem(const pdhg_parameters_t &params, const qp_problem_t &p) {
+ScaledProblem scale_problem(const pdhg_parameters_t &params, const qp_problem_t &p, MatrixPasses *dev) {
     const auto t0 = std::chrono::steady_clock::now();
     ScaledProblem w;
     w.n = p.num_variables;
@@ -352,13 +365,18 @@ ScaledProblem scale_problem(const pdhg_parameters_t &params, const qp_problem_t 
 
     w.con_rescale.assign((size_t)w.m, 1.0);
     w.var_rescale.assign((size_t)w.n, 1.0);
-    Scaler s{w, &p.cones, &p.affine_cones};
+    Scaler s{w, &p.cones, &p.affine_cones, nullptr};
     const bool preserve = params.use_cone_preserving_scaling;
     if (params.curtis_reid_iterations > 0) s.curtis_reid(params.curtis_reid_iterations, preserve);
+    if (dev) {
+        dev->load(w);
+        s.dev = dev;
+    }
     if (params.l_inf_ruiz_iterations > 0) s.ruiz(params.l_inf_ruiz_iterations, preserve);
     if (params.has_pock_chambolle_alpha) s.pock_chambolle(params.pock_chambolle_alpha, preserve);
     if (params.bound_objective_rescaling) s.bound_objective();
     else w.cbr = w.ovr = 1.0;
+    if (dev) dev->finish(w);
     if (p.cones.is_fixed)
         for (int j = 0; j < w.n; ++j)
             if (p.cones.is_fixed[j]) {

@@ -12,13 +12,14 @@
 // step costs a memory round trip): each step's A and B tiles go global ->
 // registers (16-byte loads along the contiguous dimension) -> LDS
 // (double-buffered, one barrier per step), and each wave multiplies its
-// 32 x 32 quarter as 2 x 2 blocks of v_mfma_f32_16x16x16bf16.  Per lane, the
-// A operand holds op(A)[r][k + 4g .. k + 4g + 3] for r = lane % 16,
-// g = lane / 16, the B operand op(B)[k + 4g .. + 3][c] for c = lane % 16, and
+// 32 x 32 quarter as 2 x 2 blocks of gfx950's v_mfma_f32_16x16x32_bf16 (K = 32
+// per instruction, twice the K of the CDNA3 16x16x16 form).  Per lane, the A
+// operand holds op(A)[r][k + 8g .. k + 8g + 7] for r = lane % 16,
+// g = lane / 16, the B operand op(B)[k + 8g .. + 7][c] for c = lane % 16, and
 // the 16 x 16 result holds C[4g + i][lane % 16] in register i.  An operand
-// whose storage is contiguous along k is kept k-major in LDS (one 8-byte read
-// per operand); the others are kept in their storage order and read as four
-// 2-byte values.
+// whose storage is contiguous along k is kept k-major in LDS (one 16-byte
+// read per operand); the others are kept in their storage order and read as
+// eight 2-byte values.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,7 +36,8 @@ namespace pdhcg {
 namespace mlp {
 
 typedef unsigned short bf16;
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 with round to nearest even (NaN kept quiet)
@@ -119,12 +121,13 @@ template <bool KMAJ> __device__ inline void store_tile(bf16 *S, const TileRegs &
         *reinterpret_cast<uint4 *>(d) = r.v[u];
     }
 }
-// MFMA operand for tile-local index i (row of op(A) / column of op(B)) and k-group base kb.
-template <bool KMAJ> __device__ inline s4 read_op(const bf16 *S, int i, int kb) {
-    if (KMAJ) return *reinterpret_cast<const s4 *>(S + i * kPitchK + kb);
-    s4 v;
+// MFMA operand (8 k-values) for tile-local index i (row of op(A) / column of
+// op(B)) and k-group base kb.
+template <bool KMAJ> __device__ inline s8 read_op(const bf16 *S, int i, int kb) {
+    if (KMAJ) return *reinterpret_cast<const s8 *>(S + i * kPitchK + kb);
+    s8 v;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = (short)S[(kb + q) * kPitchMN + i];
+    for (int q = 0; q < 8; ++q) v[q] = (short)S[(kb + q) * kPitchMN + i];
     return v;
 }
 
@@ -137,7 +140,7 @@ template <bool TA, bool TB, int EPI> __global__ void __launch_bounds__(256) gemm
     __shared__ float scol[2][kBN];
     __shared__ double sloss[4];
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const int l16 = lane % 16, kg = 4 * (lane / 16);
+    const int l16 = lane % 16, kg = 4 * (lane / 16), kop = 8 * (lane / 16);
     const int r0 = blockIdx.y * kBM, c0 = blockIdx.x * kBN;
     const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;
     const int nk = (g.K + kBK - 1) / kBK;
@@ -160,17 +163,18 @@ template <bool TA, bool TB, int EPI> __global__ void __launch_bounds__(256) gemm
             load_tile<BK>(rb, g.B, g.ldb, c0, g.N, (kt + 1) * kBK, g.K);
         }
 #pragma unroll
-        for (int kk = 0; kk < kBK; kk += 16) {
-            s4 a[2], b[2];
+        for (int kk = 0; kk < kBK; kk += 32) {
+            s8 a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = read_op<AK>(sA[cur], wr + 16 * i + l16, kk + kg);
+            for (int i = 0; i < 2; ++i) a[i] = read_op<AK>(sA[cur], wr + 16 * i + l16, kk + kop);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = read_op<BK>(sB[cur], wc + 16 * j + l16, kk + kg);
+            for (int j = 0; j < 2; ++j) b[j] = read_op<BK>(sB[cur], wc + 16 * j + l16, kk + kop);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a[i]),
+                                                                        __builtin_bit_cast(b8, b[j]), acc[i][j], 0, 0, 0);
         }
         if (kt + 1 < nk) {
             store_tile<AK>(sA[cur ^ 1], ra);

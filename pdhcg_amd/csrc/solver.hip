@@ -34,6 +34,10 @@ namespace pdhcg {
 // src/utils.cu:252-256).
 volatile int g_pdhcg_cancel = 0;
 
+// Problems with at least this many stored matrix entries (A, Q and R) are
+// scaled with the entry-wise passes on the device (scale.hip).
+constexpr long long kDeviceScaleMinEntries = 1LL << 20;
+
 namespace {
 
 int print_frequency(int iter) {  // reference src/utils.cu:535-546
@@ -578,7 +582,8 @@ template <class V, class X> class Solver final : public SolverBase {
     // FP32 iterates stop improving at a floor set by their rounding (a
     // relative KKT error near 1e-6 on problems whose |A||x| is large next to
     // the right-hand sides).  When the FP64-evaluated error has not halved
-    // for stall_iters_ iterations, the state moves to the evaluation copy,
+    // for stall_iters_ iterations nor for half of the iterations so far, the
+    // state moves to the evaluation copy,
     // which finishes the solve with FP64 iterates (promoted_).  The inner
     // BB solve meets the same floor sooner: its tolerance follows the primal
     // movement down to 1e-9, below what FP32 steps can resolve, and every
@@ -595,6 +600,14 @@ template <class V, class X> class Solver final : public SolverBase {
     bool is_lp_ = true;
     double cbr_ = 1, ovr_ = 1, obj_const_ = 0, obj_norm_ = 0, bound_norm_ = 0, rescale_time_ = 0;
     double setup_seconds_ = 0;
+    // setup phases (name, seconds), printed at verbose >= 1
+    std::vector<std::pair<const char *, double>> laps_;
+    std::chrono::steady_clock::time_point lap_t_;
+    void lap(const char *name) {
+        const auto now = std::chrono::steady_clock::now();
+        laps_.emplace_back(name, std::chrono::duration<double>(now - lap_t_).count());
+        lap_t_ = now;
+    }
 
     BinnedCsr<V> A_, At_, Q_, Rc_, Rtc_;
     PanelSpmv<V, Acc> pnA_, pnAt_;  // column-panel forms, built when x is larger than an L2
@@ -697,6 +710,7 @@ template <class V, class X>
 Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, hipStream_t shared, bool eval_only)
     : P_(params) {
     const auto t0 = std::chrono::steady_clock::now();
+    lap_t_ = t0;
     eval_only_ = eval_only;
     if (shared) {
         st_ = shared;
@@ -721,6 +735,7 @@ Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, h
     for (hipEvent_t &e : bb_ev_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     upload(sp);
     if (eval_only_) {
+        lap("vectors and cones");
         HIP_OK(hipStreamSynchronize(st_));
         return;
     }
@@ -730,6 +745,7 @@ Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, h
         // error before the FP64 finish (0: never)
         const char *e = std::getenv("PDHCG_MIXED_STALL");
         stall_iters_ = e ? std::atoi(e) : std::max(1000, 5 * P_.termination_evaluation_frequency);
+        lap("FP64 evaluation copy");
     }
 
     // quadratic term information (reference src/solver_state.cu:380-420)
@@ -747,6 +763,7 @@ Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, h
     }
     init_inner(sp);
     if (nonconvexity_ < 0) inner_limit_ = 1;
+    lap("Q estimates and inner solver");
     // step size and primal weight (reference src/pdhg_core_op.cu:933-971)
     if (A_.nnz == 0) step_ = 1.0;
     else {
@@ -758,8 +775,14 @@ Solver<V, X>::Solver(const ScaledProblem &sp, const pdhg_parameters_t &params, h
     graph_on_ = graph_window_default() && closed_prox() && !has_cones_ && !has_affine_;
     if (graph_on_) dyn_ = dzeros<double>(4);
     HIP_OK(hipStreamSynchronize(st_));
+    lap("step size");
     t_start_ = std::chrono::steady_clock::now();
     setup_seconds_ = std::chrono::duration<double>(t_start_ - t0).count();
+    if (P_.verbose >= 1) {
+        std::fprintf(stderr, "[setup] host scaling %.3fs, device setup %.3fs:", sp.rescale_time, setup_seconds_);
+        for (const auto &l : laps_) std::fprintf(stderr, " %s %.3fs;", l.first, l.second);
+        std::fprintf(stderr, "\n");
+    }
 }
 
 template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
@@ -775,11 +798,18 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         if (panel_mode == 1) return rows > 0;
         return (size_t)cols * sizeof(X) > kPanelMinBytes && nnz >= 4 * rows;
     };
+    lap("A binned");
     if (want_panels(n_, sp.A.nnz(), m_)) pnA_.build(m_, n_, sp.A.rp, sp.A.ci, sp.A.v);
+    lap("A column panels");
     {
-        const HostCSR t = transpose(sp.A);
+        // the device scaling hands over the transpose it built
+        HostCSR own;
+        if (sp.At.rp.empty()) own = transpose(sp.A);
+        const HostCSR &t = sp.At.rp.empty() ? own : sp.At;
+        lap("A transposed");
         At_ = upload_csr<V>(t);
         if (want_panels(m_, t.nnz(), n_)) pnAt_.build(n_, m_, t.rp, t.ci, t.v);
+        lap("A' binned");
     }
     if (qtype_ == PDHCG_SPARSE_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
         bool diag_only = true;
@@ -827,6 +857,7 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         if (sp.dkind == MiddleKind::Dense) ddense_ = dupload<double>(sp.ddense, sp.ddense.size());
         u_ = dzeros<double>((size_t)k_);
     }
+    lap("Q and R");
     lb_ = dupload<X>(sp.lb, n_);
     ub_ = dupload<X>(sp.ub, n_);
     c_ = dupload<X>(sp.c, n_);
@@ -882,7 +913,11 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
     dx_ = dzeros<X>(n_);
     dy_ = dzeros<X>(m_);
     qx_ = dzeros<X>(n_);
-    if (!eval_only_) tune_spmv();
+    if (!eval_only_) {
+        lap("vectors and cones");
+        tune_spmv();
+        lap("SpMV tuning");
+    }
 }
 
 // Mean duration (ms) of a launch sequence on the solver stream: one
@@ -1000,23 +1035,29 @@ template <class V, class X> void Solver<V, X>::init_inner(const ScaledProblem &s
         for (int i = 0; i < n_; ++i)
             for (int e = sp.Q.rp[i]; e < sp.Q.rp[(size_t)i + 1]; ++e)
                 if (sp.Q.ci[e] == i) dh[i] += sp.Q.v[e];
-    if (qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+    if ((qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) && sp.dkind == MiddleKind::Dense) {
         const HostCSR rt = transpose(sp.R);
         for (int i = 0; i < n_; ++i) {
             double acc = 0.0;
             const int e0 = rt.rp[i], e1 = rt.rp[(size_t)i + 1];
-            if (sp.dkind == MiddleKind::Diag) {
-                for (int e = e0; e < e1; ++e) acc += sp.ddiag[rt.ci[e]] * rt.v[e] * rt.v[e];
-            } else if (sp.dkind == MiddleKind::Dense) {
-                for (int a = e0; a < e1; ++a) {
-                    const double *Drow = sp.ddense.data() + (size_t)rt.ci[a] * k_;
-                    for (int b = e0; b < e1; ++b) acc += rt.v[a] * rt.v[b] * Drow[rt.ci[b]];
-                }
-            } else {
-                for (int e = e0; e < e1; ++e) acc += rt.v[e] * rt.v[e];
+            for (int a = e0; a < e1; ++a) {
+                const double *Drow = sp.ddense.data() + (size_t)rt.ci[a] * k_;
+                for (int b = e0; b < e1; ++b) acc += rt.v[a] * rt.v[b] * Drow[rt.ci[b]];
             }
             dh[i] += acc;
         }
+    } else if (qtype_ == PDHCG_LOW_RANK_Q || qtype_ == PDHCG_LOW_RANK_PLUS_SPARSE_Q) {
+        // diag(R'DR) for diagonal (or no) D, accumulated row by row of R: each
+        // column receives its terms in increasing row order, as a pass over
+        // the transpose would add them, without building the transpose
+        std::vector<double> acc((size_t)n_, 0.0);
+        const bool dg = sp.dkind == MiddleKind::Diag;
+        for (int j = 0; j < k_; ++j)
+            for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e) {
+                const double v = sp.R.v[e];
+                acc[sp.R.ci[e]] += dg ? sp.ddiag[j] * v * v : v * v;
+            }
+        for (int i = 0; i < n_; ++i) dh[i] += acc[i];
     }
     diag_h_ = dupload<double>(dh, n_);
     mdiag_ = dzeros<X>(n_);
@@ -1560,7 +1601,9 @@ template <class V, class X> bool Solver<V, X>::stalled(int tc) {
         best_err_iter_ = tc;
         return false;
     }
-    return tc - best_err_iter_ >= stall_iters_;
+    // a plateau between restarts is not a stall: the error must also have
+    // stood still for half of the run so far
+    return tc - best_err_iter_ >= std::max(stall_iters_, tc / 2);
 }
 
 // Hands the loop state to the FP64 evaluation copy: the iterates widened
@@ -1818,8 +1861,8 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
         if (eval_ && stall_iters_ > 0 && sat_run_ >= kSatPromote) {
             promote();
             if (P_.verbose >= 1)
-                std::printf("[mixed] %d FP32 inner solves in a row reached the inner limit; FP64 iterates from "
-                            "iteration %d\n", sat_run_, tc);
+                std::fprintf(stderr, "[mixed] %d FP32 inner solves in a row reached the inner limit; FP64 iterates "
+                             "from iteration %d\n", sat_run_, tc);
             return eval_->run(iterations - done);
         }
         if (is_major_ || tc == 0 || tc % print_frequency(tc) == 0) {
@@ -1835,8 +1878,8 @@ template <class V, class X> int Solver<V, X>::run(int iterations) {
             if (eval_ && stalled(tc)) {
                 promote();
                 if (P_.verbose >= 1)
-                    std::printf("[mixed] FP32 iterates stalled at a KKT error of %.1e; FP64 iterates from iteration %d\n",
-                                best_err_, tc);
+                    std::fprintf(stderr, "[mixed] FP32 iterates stalled at a KKT error of %.1e; FP64 iterates from "
+                                 "iteration %d\n", best_err_, tc);
                 return eval_->run(iterations - done);
             }
         }
@@ -2018,7 +2061,18 @@ pdhcg_solver_t *pdhcg_solver_create(const qp_problem_t *prob, const pdhg_paramet
                              s->lifted->constraint_matrix_num_nonzeros);
             s->problem = s->lifted;
         }
-        ScaledProblem sp = scale_problem(s->params, *s->problem);
+        // the entry-wise scaling passes and the transpose of A run on the
+        // device for problems of at least kDeviceScaleMinEntries stored
+        // entries (PDHCG_DEVICE_SCALING=0 keeps them on the host, =1 forces
+        // the device); both give the same scaled problem bit for bit
+        const qp_problem_t &q = *s->problem;
+        const long long entries = (long long)q.constraint_matrix_num_nonzeros +
+                                  q.objective_sparse_matrix_num_nonzeros + q.objective_lowrank_matrix_num_nonzeros;
+        const char *ds = std::getenv("PDHCG_DEVICE_SCALING");
+        const bool on_device = ds ? std::atoi(ds) != 0 : entries >= kDeviceScaleMinEntries;
+        std::unique_ptr<MatrixPasses> passes = on_device ? make_device_passes() : nullptr;
+        ScaledProblem sp = scale_problem(s->params, q, passes.get());
+        passes.reset();
         if (precision == PDHCG_PRECISION_MIXED) s->impl.reset(new Solver<float, float>(sp, s->params));
         else s->impl.reset(new Solver<double, double>(sp, s->params));
         return s;

@@ -168,5 +168,26 @@ class WarmStart:
         x = np.clip(x[0], lb, ub)
         return replace(problem, primal_start=x, dual_start=y[0] if dual else None)
 
+    def guarded(self, problems, Z, dual: bool = True, batch: bool = True, **params):
+        """Predicted starts with the guard: instance i keeps its predicted
+        start only where the relative KKT error at that point (the largest of
+        the relative primal, dual and gap residuals the termination test
+        uses) is below the cold start's; otherwise it starts cold.  Both
+        errors come from the device solver at iteration 0 (iteration_limit
+        0), batched when ``batch``.  Returns (problems, kept) with kept[i]
+        True where the predicted start was kept."""
+        from . import solve, solve_batch
+        Z = np.asarray(Z, dtype=np.float32).reshape(len(problems), -1)
+        warm = [self.start(p, z, dual) for p, z in zip(problems, Z)]
+        kw = dict(params, iteration_limit=0)
+        if batch:
+            r = solve_batch(list(problems) + warm, **kw)
+        else:
+            r = [solve(p, **kw) for p in list(problems) + warm]
+        err = lambda s_: np.inf if s_ is None else max(s_.rel_primal, s_.rel_dual, s_.rel_gap)
+        count = len(problems)
+        kept = np.array([err(r[count + i]) < err(r[i]) for i in range(count)], dtype=bool)
+        return [warm[i] if kept[i] else problems[i] for i in range(count)], kept
+
     def close(self):
         self.net.close()
