@@ -223,17 +223,38 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
            "single_device_ms_per_problem": single_ms, "single_sample": len(sample),
            "speedup_vs_single_device": single_ms / (1e3 * batch_s / count),
            "note": "time_to_tol_s covers host scaling and packing plus the device solve of the whole batch"}
+    # the mixed form: FP32 iterates and matrix values, then every problem
+    # evaluated (and if need be finished) in FP64
+    t = time.perf_counter()
+    msols = pdhcg_amd.solve_batch(problems, precision=pdhcg_amd.PRECISION_MIXED, eps=eps, iteration_limit=100000,
+                                  time_limit=300.0, verbose=1)
+    mixed_s = time.perf_counter() - t
+    mstatus = {}
+    for s_ in msols:
+        k = s_.status if s_ is not None else "REJECTED"
+        mstatus[k] = mstatus.get(k, 0) + 1
+    miters = [s_.iterations for s_ in msols if s_ is not None]
+    rel = [abs(a.pobj - b.pobj) / (1.0 + abs(b.pobj)) for a, b in zip(msols, sols) if a is not None and b is not None]
+    out["mixed"] = {"time_to_tol_s": mixed_s, "ms_per_problem": 1e3 * mixed_s / count, "status": mstatus,
+                    "iterations_mean": float(np.mean(miters)), "iterations_max": int(max(miters)),
+                    "max_rel_objective_diff_vs_fp64": float(max(rel)) if rel else None,
+                    "speedup_vs_fp64_batch": batch_s / mixed_s,
+                    "speedup_vs_single_device": single_ms / (1e3 * mixed_s / count)}
+    log(f"[bench] batch mixed: solved in {mixed_s:.2f}s, {mstatus}")
     if with_cpu:
         from tests.oracle_lib import oracle_solve
         t = time.perf_counter()
-        done, same_status, obj_diff = 0, 0, 0.0
+        done, same_status, obj_diff, m_same, m_diff = 0, 0, 0.0, 0, 0.0
         for i in range(0, count, 64):
             o = oracle_solve(problems[i], eps=eps, iteration_limit=100000)
             done += 1
-            g = sols[i]
+            g, gm = sols[i], msols[i]
             if g is not None and o is not None and g.status == o.status:
                 same_status += 1
                 obj_diff = max(obj_diff, abs(g.pobj - o.pobj) / (1.0 + abs(o.pobj)))
+            if gm is not None and o is not None and gm.status == o.status:
+                m_same += 1
+                m_diff = max(m_diff, abs(gm.pobj - o.pobj) / (1.0 + abs(o.pobj)))
             if time.perf_counter() - t > cpu_seconds:
                 break
         cpu_ms = 1e3 * (time.perf_counter() - t) / max(done, 1)
@@ -242,6 +263,7 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
         # the oracle's solves of those instances against the batch's results
         out["oracle_check"] = {"instances": done, "same_status": same_status,
                                "max_rel_objective_diff": obj_diff}
+        out["mixed"]["oracle_check"] = {"instances": done, "same_status": m_same, "max_rel_objective_diff": m_diff}
     return out
 
 
@@ -299,7 +321,7 @@ def run_warm_large(args):
     """The warm start on the family at the stated instance size: n = 1e5,
     m = 5e4, nnz(A) = 1e6, rank 50 (network 16 -> 1024 -> 1024 -> 150000)."""
     return run_warm(args.warm_large_train, args.warm_large_test, args.eps, n=100_000, m=50_000, nnz=1_000_000,
-                    singles=2, epochs=200, tols=(args.eps,))
+                    singles=2, epochs=200, tols=(args.eps,), batch=64, guard=False)
 
 
 def capi_const(name):
@@ -315,7 +337,8 @@ def mlp_step_flops(in_dim, hidden, out_dim, batch):
     return 2.0 * batch * (2 * i * h + 3 * h * h + 3 * h * o)
 
 
-def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4, epochs=300, tols=None):
+def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4, epochs=300, tols=None, batch=256,
+             guard=True):
     """The learned warm start on the parametric sparse QP + low-rank family
     (n x m with nnz(A) nonzeros, rank 50, z in R^16; defaults n = 2000,
     m = 1000, nnz = 20000): training solutions from the batched solver at
@@ -339,7 +362,7 @@ def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4,
     Y = np.stack([sols[i].y for i in ok])
     del sols, ptr
     log(f"[bench] warm: {len(ok)} of {train_count} training solutions in {gen_s:.1f}s")
-    batch = 256
+    batch = max(16, min(batch, len(ok) // 16 * 16))
     ws = warmstart.WarmStart(fam.z_dim, fam.n, fam.m, hidden=1024, seed=0)
     t = time.perf_counter()
     loss = ws.fit(Ztr[ok], X, Y, epochs=epochs, batch=batch, lr=1e-3)
@@ -367,10 +390,13 @@ def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4,
         t = time.perf_counter()
         primal = pdhcg_amd.solve_batch(primal_p, eps=tol, iteration_limit=100000, time_limit=300.0)
         primal_s = time.perf_counter() - t
-        t = time.perf_counter()
-        guard_p, kept = ws.guarded(pte, Zte, eps=tol)
-        guarded = pdhcg_amd.solve_batch(guard_p, eps=tol, iteration_limit=100000, time_limit=300.0)
-        guarded_s = time.perf_counter() - t
+        if guard:
+            t = time.perf_counter()
+            guard_p, kept = ws.guarded(pte, Zte, eps=tol)
+            guarded = pdhcg_amd.solve_batch(guard_p, eps=tol, iteration_limit=100000, time_limit=300.0)
+            guarded_s = time.perf_counter() - t
+        else:
+            guard_p, kept, guarded, guarded_s = pte, np.zeros(test_count, dtype=bool), cold, None
         ig = np.array([s_.iterations for s_ in guarded])
         ip = np.array([s_.iterations for s_ in primal])
         ic = np.array([s_.iterations for s_ in cold])
@@ -407,11 +433,13 @@ def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4,
             for k in sample:
                 pdhcg_amd.solve(warm_p[k], eps=tol, iteration_limit=100000)
             single_warm = (time.perf_counter() - t) / len(sample)
-            t = time.perf_counter()
-            for k in sample:
-                gp, _ = ws.guarded([pte[k]], Zte[k:k + 1], batch=False, eps=tol)
-                pdhcg_amd.solve(gp[0], eps=tol, iteration_limit=100000)
-            single_guarded = (time.perf_counter() - t) / len(sample)
+            single_guarded = None
+            if guard:
+                t = time.perf_counter()
+                for k in sample:
+                    gp, _ = ws.guarded([pte[k]], Zte[k:k + 1], batch=False, eps=tol)
+                    pdhcg_amd.solve(gp[0], eps=tol, iteration_limit=100000)
+                single_guarded = (time.perf_counter() - t) / len(sample)
             rec["single_device_cold_s"] = single_cold
             rec["single_device_warm_s"] = single_warm
             rec["single_device_guarded_s"] = single_guarded
@@ -421,7 +449,7 @@ def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4,
             rec["speedup_batched_primal_only_vs_single_cold"] = single_cold / (primal_s / test_count)
         out["tol_%g" % tol] = rec
         log(f"[bench] warm n={fam.n}: tol {tol:g}: batch s cold {cold_s:.2f} warm {warm_s:.2f} primal-only "
-            f"{primal_s:.2f} guarded {guarded_s:.2f}; iterations cold {ic.mean():.0f} warm {iw.mean():.0f} "
+            f"{primal_s:.2f} guarded {guarded_s if guarded_s is None else round(guarded_s, 2)}; iterations cold {ic.mean():.0f} warm {iw.mean():.0f} "
             f"primal-only {ip.mean():.0f} guarded {ig.mean():.0f} (max {ic.max()}/{iw.max()}/{ip.max()}/{ig.max()})")
     ws.close()
     return out

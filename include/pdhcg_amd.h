@@ -132,8 +132,13 @@ int pdhcg_project_cones_noalloc(const cone_blocks_t *cones, const double *d, dou
  * affine PSD cones, a low-rank term above rank 256.  Affine second-order
  * cones of any length are taken (above 64 vector slots the problem's
  * workgroup projects them one at a time).  params (NULL: defaults) apply to every
- * problem; the time limit is checked between windows of iterations.  Returns
- * 0, or -1 for invalid arguments or a device error. */
+ * problem; the time limit is checked between windows of iterations.  With
+ * pdhcg_set_precision(PDHCG_PRECISION_MIXED) the problems first iterate with
+ * FP32 iterates and matrix values (FP64 reductions and scalars) until each
+ * terminates or stops improving at the FP32 rounding floor; the iterates are
+ * then widened and every problem is evaluated, and if need be finished, in
+ * FP64, so every status and residual is an FP64 one.  Returns 0, or -1 for
+ * invalid arguments or a device error. */
 int pdhcg_solve_batch(const qp_problem_t *const *problems, int count, const pdhg_parameters_t *params,
                       pdhcg_result_t **results);
 

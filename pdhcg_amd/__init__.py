@@ -86,11 +86,13 @@ def set_graph_window(enable: bool) -> None:
     lib().pdhcg_set_graph_window(1 if enable else 0)
 
 
-def solve_batch(problems, **params) -> list:
+def solve_batch(problems, precision: int = PRECISION_FP64, **params) -> list:
     """Solves the problems together through pdhcg_solve_batch (one device
     workgroup per problem); returns one Solution per problem, or None for a
-    problem the batch rejected."""
+    problem the batch rejected.  PRECISION_MIXED runs FP32 iterates first and
+    evaluates and terminates every problem in FP64."""
     L = lib()
+    L.pdhcg_set_precision(precision)
     handles = [build(L, "", p) for p in problems]
     try:
         count = len(problems)

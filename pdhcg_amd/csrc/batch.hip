@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -30,13 +31,53 @@ namespace {
 // Loop state of one problem, persisted between launches (POD, same layout
 // on host and device).
 struct HostState {
-    int total, inner_count, inner_total, is_major, do_restart, reason;
+    int total, inner_count, inner_total, is_major, do_restart, reason, best_iter, sat, eval_now, pad;
+    double best_err;
     double pw, pw_err_sum, pw_last_err, best_pw, best_gap;
     double fpe, init_fpe, last_trial_fpe, inner_tol, cached_inv_tau, tol_scale;
     double abs_pr, abs_du, rel_pr, rel_du, pobj, dobj, gap, rel_gap;
     double max_pray, max_dray, pray_lin_obj, dray_obj;
 };
 static_assert(sizeof(HostState) == sizeof(State), "host and device loop state differ");
+
+inline int grid_of(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 8192)); }
+
+__global__ void narrow_kernel(const double *a, float *b, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+// The iterate and work vectors of problem blockIdx.x widened from the FP32
+// slab into the FP64 slab; the problem data (c, bounds, scalings, the
+// Jacobi diagonal, g, row bounds) keep their FP64 values.
+__global__ void widen_iterates_kernel(const Inst *insts, const float *s32, double *s64) {
+    const Inst I = insts[blockIdx.x];
+    const int nv[] = {vX0, vX, vXP, vXR, vATY, vDS, vGR, vDIR, vQX, vPB, vDX, vMD, vMI};
+    const int mv[] = {wY0, wY, wYP, wYR, wAX, wDY, wPRES, wPSL, wAPR, wAMEM};
+    for (int w : nv)
+        for (int i = threadIdx.x; i < I.n; i += blockDim.x) {
+            const long long o = I.vec + (long long)w * I.n + i;
+            s64[o] = (double)s32[o];
+        }
+    for (int w : mv)
+        for (int i = threadIdx.x; i < I.m; i += blockDim.x) {
+            const long long o = I.vec + (long long)kNVec * I.n + (long long)w * I.m + i;
+            s64[o] = (double)s32[o];
+        }
+}
+
+// Every problem that left the FP32 phase (terminated there, stalled or at
+// the iteration limit) is evaluated again by the FP64 kernel at its next
+// pass, which decides its status; the Jacobi metric is rebuilt from the
+// FP64 diagonal.
+__global__ void resume_fp64_kernel(State *states, int count) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < count; b += gridDim.x * blockDim.x) {
+        State &S = states[b];
+        S.reason = TERMINATION_REASON_UNSPECIFIED;
+        S.eval_now = 1;
+        S.cached_inv_tau = -1.0;
+    }
+}
 
 // sigma_max(A) by the power iteration on A A' the single-problem solver uses
 // (reference src/pdhg_core_op.cu:1673-1788): start rand_r(1234) in [-1, 1].
@@ -79,7 +120,8 @@ struct Packed {
     ScaledProblem sp;
     std::vector<int> ai;     // A rp, A ci, At rp, At ci
     std::vector<double> av;  // A v, At v, R dense, D
-    std::vector<double> slab;
+    std::vector<double> slab;   // n- and m-vectors
+    std::vector<double> wslab;  // k-vectors and per-cone values
     std::vector<Cone> cones;
     HostState st;
 };
@@ -206,7 +248,8 @@ bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const
     }
     I.pw0 = P.bound_objective_rescaling ? 1.0 : (sp.obj_norm + 1.0) / (sp.bound_norm + 1.0);
     // the slab (FP64), starting point projected onto the box
-    k.slab.assign((size_t)slab_size(n, m, I.k, I.ncones), 0.0);
+    k.slab.assign((size_t)slab_size(n, m), 0.0);
+    k.wslab.assign((size_t)wslab_size(I.k, I.ncones), 0.0);
     auto N = [&](int w) { return k.slab.data() + (size_t)w * n; };
     auto M = [&](int w) { return k.slab.data() + (size_t)kNVec * n + (size_t)w * m; };
     for (int i = 0; i < n; ++i) {
@@ -238,6 +281,7 @@ bool pack(const qp_problem_t *prob, const pdhg_parameters_t &P, Packed &k, const
     S.pw = S.best_pw = I.pw0;
     S.best_gap = INFINITY;
     S.last_trial_fpe = INFINITY;
+    S.best_err = INFINITY;
     S.inner_tol = P.inner_solver_parameters.initial_tolerance > 0 ? P.inner_solver_parameters.initial_tolerance : 1e-3;
     S.cached_inv_tau = -1.0;
     S.tol_scale = 1.0;
@@ -333,18 +377,20 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         std::vector<Inst> insts((size_t)B);
         std::vector<HostState> states((size_t)B);
         std::vector<long long> ioff((size_t)B + 1, 0), voff((size_t)B + 1, 0), coff((size_t)B + 1, 0),
-            soff((size_t)B + 1, 0);
+            soff((size_t)B + 1, 0), woff((size_t)B + 1, 0);
         for (int b = 0; b < B; ++b) {
             ioff[b + 1] = ioff[b] + (long long)pk[b].ai.size();
             voff[b + 1] = voff[b] + (long long)pk[b].av.size();
             coff[b + 1] = coff[b] + (long long)pk[b].cones.size();
             soff[b + 1] = soff[b] + (long long)pk[b].slab.size();
+            woff[b + 1] = woff[b] + (long long)pk[b].wslab.size();
         }
         if (ioff[B] > (long long)INT32_MAX) fail("the batch's index pool exceeds 2^31 entries");
         auto ipool = raw_buffer<int>((size_t)ioff[B]);
         auto vpool = raw_buffer<double>((size_t)voff[B]);
         auto cones = raw_buffer<Cone>((size_t)coff[B]);
         auto slabs = raw_buffer<double>((size_t)soff[B]);
+        auto wslabs = raw_buffer<double>((size_t)woff[B]);
         std::atomic<bool> pack_error{false};
         parallel_for(B, [&](int b) {
             Packed &q = pk[b];
@@ -361,10 +407,12 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             I.dmat += vo;
             I.cone0 = (int)coff[b];
             I.vec = soff[b];
+            I.wvec = woff[b];
             std::copy(q.ai.begin(), q.ai.end(), ipool.get() + ioff[b]);
             std::copy(q.av.begin(), q.av.end(), vpool.get() + vo);
             std::copy(q.cones.begin(), q.cones.end(), cones.get() + coff[b]);
             std::copy(q.slab.begin(), q.slab.end(), slabs.get() + soff[b]);
+            std::copy(q.wslab.begin(), q.wslab.end(), wslabs.get() + woff[b]);
             // every offset the kernel derives from I stays inside this
             // problem's share of the pools (a failure is a packing bug)
             const long long nnz = (long long)q.inst.at_rp - q.inst.a_ci;
@@ -373,7 +421,8 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                       I.at_ci == I.at_rp + I.n + 1 && I.at_ci + nnz <= ioff[b + 1] && rp[0] == 0 &&
                       rp[I.m] == nnz && trp[0] == 0 && trp[I.n] == nnz && I.at_v == I.a_v + nnz &&
                       I.rdense == I.at_v + nnz && I.dmat >= I.rdense && I.dmat <= voff[b + 1] &&
-                      I.vec + slab_size(I.n, I.m, I.k, I.ncones) <= soff[b + 1] && I.cone0 + I.ncones <= coff[b + 1];
+                      I.vec + slab_size(I.n, I.m) <= soff[b + 1] && I.wvec + wslab_size(I.k, I.ncones) <= woff[b + 1] &&
+                      I.cone0 + I.ncones <= coff[b + 1];
             for (long long e = 0; ok && e < nnz; ++e) {
                 const int c = ipool[(size_t)I.a_ci + e], r = ipool[(size_t)I.at_ci + e];
                 ok = c >= 0 && c < I.n && r >= 0 && r < I.m;
@@ -392,6 +441,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             std::vector<int>().swap(q.ai);
             std::vector<double>().swap(q.av);
             std::vector<double>().swap(q.slab);
+            std::vector<double>().swap(q.wslab);
         });
         if (pack_error.load()) fail("packing produced an offset outside a problem's pools (internal error)");
         Params dp;
@@ -414,6 +464,11 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         dp.suff = P.restart_params.sufficient_reduction_for_restart;
         dp.nec = P.restart_params.necessary_reduction_for_restart;
         dp.art = P.restart_params.artificial_restart_threshold;
+        {  // iterations without a halving of the KKT error before a problem leaves the FP32 phase
+            const char *e = std::getenv("PDHCG_MIXED_STALL");
+            dp.stall_iters = e ? std::atoi(e) : std::max(1000, 5 * dp.ef);
+        }
+        const bool mixed = pdhcg_get_precision() == PDHCG_PRECISION_MIXED;
         auto track = [&](void *p) {
             dev.push_back(p);
             return p;
@@ -423,6 +478,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         int *d_ipool = (int *)track(upload(ipool.get(), (size_t)ioff[B]));
         double *d_vpool = (double *)track(upload(vpool.get(), (size_t)voff[B]));
         double *d_slabs = (double *)track(upload(slabs.get(), (size_t)soff[B]));
+        double *d_wslabs = (double *)track(upload(wslabs.get(), (size_t)woff[B]));
         Cone *d_cones = (Cone *)track(upload(cones.get(), (size_t)coff[B]));
         State *d_states = (State *)track(upload(states));
         Out *d_out = (Out *)track(upload(std::vector<Out>((size_t)B)));
@@ -435,37 +491,90 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         } sg;
         HIP_OK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
         hipStream_t st = sg.s;
+        // mixed precision: FP32 copies of the matrix values and of the slabs
+        float *d_vpool32 = nullptr, *d_slabs32 = nullptr;
+        if (mixed) {
+            HIP_OK(hipMalloc(&d_vpool32, std::max<size_t>((size_t)voff[B], 1) * sizeof(float)));
+            dev.push_back(d_vpool32);
+            HIP_OK(hipMalloc(&d_slabs32, std::max<size_t>((size_t)soff[B], 1) * sizeof(float)));
+            dev.push_back(d_slabs32);
+            hipLaunchKernelGGL(narrow_kernel, dim3(grid_of(voff[B])), dim3(256), 0, st, (const double *)d_vpool,
+                               d_vpool32, voff[B]);
+            hipLaunchKernelGGL(narrow_kernel, dim3(grid_of(soff[B])), dim3(256), 0, st, (const double *)d_slabs,
+                               d_slabs32, soff[B]);
+            HIP_OK(hipGetLastError());
+        }
         const auto t_dev = std::chrono::steady_clock::now();
         int windows = 0;
         // windows of iterations; every problem that has terminated returns at once
         const int window = std::max(1, std::min(dp.ef * 16, 2000));
         std::vector<Out> out((size_t)B);
         bool time_up = false, cancelled = false;
-        for (;;) {
-            const auto tw = std::chrono::steady_clock::now();
-            hipLaunchKernelGGL(batch_solve_kernel, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool, d_vpool,
-                               d_slabs, d_cones, d_states, d_out, window);
-            ++windows;
+        // runs windows until no problem is running (or time / cancel); `budget`
+        // passes per window; `phase` names the kernel in the progress lines
+        auto run_phase = [&](auto launch, const char *phase, int max_windows) {
+            for (int w = 0; w < max_windows; ++w) {
+                const auto tw = std::chrono::steady_clock::now();
+                launch();
+                ++windows;
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(Out), hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                int running = 0;
+                for (const Out &o : out) running += o.status == TERMINATION_REASON_UNSPECIFIED ? 1 : 0;
+                if (P.verbose >= 1)  // the tail: how many problems each window leaves running
+                    std::fprintf(stderr, "[batch] %s window %d (%d passes): %.3fs, %d of %d problems still running\n",
+                                 phase, windows, window,
+                                 std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count(), running,
+                                 B);
+                if (running == 0) return;
+                if (g_pdhcg_cancel) {  // user interrupt, checked between windows
+                    cancelled = true;
+                    return;
+                }
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (el >= P.termination_criteria.time_sec_limit) {
+                    time_up = true;
+                    return;
+                }
+            }
+        };
+        if (mixed) {
+            run_phase(
+                [&] {
+                    hipLaunchKernelGGL(batch_solve_kernel<float>, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool,
+                                       (const float *)d_vpool32, (const double *)d_vpool, d_slabs32, d_wslabs, d_cones,
+                                       d_states, d_out, window);
+                },
+                "FP32", INT_MAX);
+            // FP32 -> FP64: the iterates widened exactly into the FP64 slabs
+            // (the FP64 problem data stay), and every problem that has not
+            // hit a limit evaluated again in FP64 before it may terminate
+            hipLaunchKernelGGL(widen_iterates_kernel, dim3(B), dim3(256), 0, st, (const Inst *)d_insts,
+                               (const float *)d_slabs32, d_slabs);
+            hipLaunchKernelGGL(resume_fp64_kernel, dim3(grid_of(B)), dim3(256), 0, st, d_states, B);
             HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(Out), hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            int running = 0;
-            for (const Out &o : out) running += o.status == TERMINATION_REASON_UNSPECIFIED ? 1 : 0;
-            if (P.verbose >= 1)  // the tail: how many problems each window leaves running
-                std::fprintf(stderr, "[batch] window %d (%d passes): %.3fs, %d of %d problems still running\n", windows,
-                             window, std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count(),
-                             running, B);
-            const bool all = running == 0;
-            if (all) break;
-            if (g_pdhcg_cancel) {  // user interrupt, checked between windows
-                cancelled = true;
-                break;
+            if (P.verbose >= 1) {
+                int stalled = 0;
+                for (const Out &o : out) stalled += o.status == kStalled ? 1 : 0;
+                std::fprintf(stderr, "[batch] FP32 phase done: %d of %d problems left it at the FP32 floor\n",
+                             stalled, B);
             }
-            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (el >= P.termination_criteria.time_sec_limit) {
-                time_up = true;
-                break;
-            }
+        }
+        // FP64; after a timed-out or cancelled FP32 phase, one pass that
+        // evaluates every problem in FP64
+        const bool stop = time_up || cancelled;
+        const bool was_time_up = time_up, was_cancelled = cancelled;
+        run_phase(
+            [&] {
+                hipLaunchKernelGGL(batch_solve_kernel<double>, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool,
+                                   (const double *)d_vpool, (const double *)d_vpool, d_slabs, d_wslabs, d_cones,
+                                   d_states, d_out, stop ? 1 : window);
+            },
+            "FP64", stop ? 1 : INT_MAX);
+        if (stop) {
+            time_up = was_time_up;
+            cancelled = was_cancelled;
         }
         const auto t_end = std::chrono::steady_clock::now();
         const double elapsed = std::chrono::duration<double>(t_end - t0).count();

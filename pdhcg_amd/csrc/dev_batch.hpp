@@ -21,6 +21,7 @@
 #pragma once
 
 #include <cfloat>
+#include <type_traits>
 
 #include "dev_common.hpp"
 #include "dev_cones.hpp"
@@ -36,7 +37,7 @@ enum QKind { kQNone = 0, kQDiag = 1, kQLowRank = 2 };
 
 // Loop parameters shared by the batch (pdhg_parameters_t fields).
 struct Params {
-    int ef, iteration_limit, norm_inf, inner_limit, precond;
+    int ef, iteration_limit, norm_inf, inner_limit, precond, stall_iters;
     double eps_opt, eps_feas, eps_inf, rho;
     double inner_tol0, inner_min_tol;
     double i_smooth, k_p, k_i, k_d, suff, nec, art;
@@ -49,7 +50,8 @@ struct Inst {
     int a_rp, a_ci, at_rp, at_ci;  // int-pool offsets of [A;F] and its transpose
     int a_long, at_long;           // 1: a wavefront per row, 0: a thread per row
     long long a_v, at_v;           // value-pool offsets of the matrix values
-    long long vec;                 // vector-pool offset of the problem's slab
+    long long vec;                 // slab offset of the problem's n- and m-vectors
+    long long wvec;                // FP64 slab offset of its k-vectors and per-cone values
     long long rdense, dmat;        // value-pool offsets of R (k x n row-major) and D
     double cbr, ovr, obj_const, obj_norm, bound_norm, step, pw0, nonconvexity;
 };
@@ -66,16 +68,16 @@ struct Out {
     double max_pray, max_dray, pray_lin_obj, dray_obj;
 };
 
-// Slab layout: n-vectors, then m-vectors, then k-vectors, then per-cone values.
+// Slab layout: n-vectors, then m-vectors (storage type of the kernel); the
+// FP64 slab: k-vectors, then per-cone values.
 enum NVec { vC, vLB, vUB, vVR, vQD, vDH, vX0, vX, vXP, vXR, vATY, vDS, vGR, vDIR, vQX, vPB, vDX, vMD, vMI, vLBF, vUBF,
             kNVec };
 enum MVec { wG, wCL, wCU, wCLF, wCUF, wCR, wY0, wY, wYP, wYR, wAX, wDY, wPRES, wPSL, wAPR, wAMEM, wAFD, kMVec };
 constexpr int kKVec = 2;      // rx, rx buffer
 constexpr int kConeVals = 3;  // projection warm start, residual warm start, complementarity
 
-inline long long slab_size(int n, int m, int k, int ncones) {
-    return (long long)kNVec * n + (long long)kMVec * m + (long long)kKVec * k + (long long)kConeVals * ncones;
-}
+inline long long slab_size(int n, int m) { return (long long)kNVec * n + (long long)kMVec * m; }
+inline long long wslab_size(int k, int ncones) { return (long long)kKVec * k + (long long)kConeVals * ncones; }
 
 __device__ inline int print_frequency(int it) {
     int step = 10;
@@ -132,12 +134,13 @@ struct Red {
 // and two waves per SIMD there is little occupancy to hide memory latency,
 // so each lane keeps four independent partial sums (four loads in flight)
 // and adds them in a fixed order.
-__device__ inline void spmv(const int *rp, const int *ci, const double *v, const double *x, double *y, int rows,
+template <class T>
+__device__ inline void spmv(const int *rp, const int *ci, const T *v, const T *x, T *y, int rows,
                             int wave_rows) {
     if (wave_rows) {
         const int lane = threadIdx.x % kWave;
         for (int r = threadIdx.x / kWave; r < rows; r += kBW) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
             const int end = rp[r + 1];
             int e = rp[r] + lane;
             for (; e + 3 * kWave < end; e += 4 * kWave) {
@@ -147,12 +150,12 @@ __device__ inline void spmv(const int *rp, const int *ci, const double *v, const
                 s3 += v[e + 3 * kWave] * x[ci[e + 3 * kWave]];
             }
             for (; e < end; e += kWave) s0 += v[e] * x[ci[e]];
-            const double s = wave_sum((s0 + s1) + (s2 + s3));
+            const T s = wave_sum((s0 + s1) + (s2 + s3));
             if (lane == 0) y[r] = s;
         }
     } else {
         for (int r = threadIdx.x; r < rows; r += kBT) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
             const int end = rp[r + 1];
             int e = rp[r];
             for (; e + 3 < end; e += 4) {
@@ -168,32 +171,32 @@ __device__ inline void spmv(const int *rp, const int *ci, const double *v, const
     __syncthreads();
 }
 
-// Everything one problem's loop needs, as device pointers.
-struct Ctx {
+// Everything one problem's loop needs, as device pointers.  T is the
+// storage and product type of the iterates and matrix values (double, or
+// float for the FP32 phase of the mixed batch); the k-vectors, per-cone
+// values, D and every reduction and scalar stay FP64.
+template <class T> struct Ctx {
     const Params *P;
     Inst I;
     const int *arp, *aci, *atrp, *atci;
-    const double *av, *atv, *R, *D;
-    double *V;  // slab
+    const T *av, *atv, *R;
+    const double *D;
+    T *V;       // n- and m-vectors of the slab
+    double *W;  // k-vectors and per-cone values (FP64)
     const Cone *cones;
     Red red;
-    __device__ double *nv(int which) const { return V + (long long)which * I.n; }
-    __device__ double *mv(int which) const { return V + (long long)kNVec * I.n + (long long)which * I.m; }
-    __device__ double *kv(int which) const {
-        return V + (long long)kNVec * I.n + (long long)kMVec * I.m + (long long)which * I.k;
-    }
-    __device__ double *cv(int which) const {
-        return V + (long long)kNVec * I.n + (long long)kMVec * I.m + (long long)kKVec * I.k +
-               (long long)which * I.ncones;
-    }
-    __device__ void A(const double *x, double *y) const { spmv(arp, aci, av, x, y, I.m, I.a_long); }
-    __device__ void At(const double *y, double *x) const { spmv(atrp, atci, atv, y, x, I.n, I.at_long); }
+    __device__ T *nv(int which) const { return V + (long long)which * I.n; }
+    __device__ T *mv(int which) const { return V + (long long)kNVec * I.n + (long long)which * I.m; }
+    __device__ double *kv(int which) const { return W + (long long)which * I.k; }
+    __device__ double *cv(int which) const { return W + (long long)kKVec * I.k + (long long)which * I.ncones; }
+    __device__ void A(const T *x, T *y) const { spmv(arp, aci, av, x, y, I.m, I.a_long); }
+    __device__ void At(const T *y, T *x) const { spmv(atrp, atci, atv, y, x, I.n, I.at_long); }
 
     // qx = Q x (reference src/pdhg_core_op.cu:305-413): diagonal part plus R'DR x
-    __device__ __noinline__ void obj_product(const double *x, double *qx) const {
+    __device__ __noinline__ void obj_product(const T *x, T *qx) const {
         const int n = I.n, k = I.k;
         if (I.qkind == kQNone) return;
-        const double *qd = nv(vQD);
+        const T *qd = nv(vQD);
         if (I.qkind == kQDiag) {
             for (int i = threadIdx.x; i < n; i += kBT) qx[i] = qd[i] * x[i];
             __syncthreads();
@@ -202,8 +205,8 @@ struct Ctx {
         double *rx = kv(0), *rb = kv(1);
         const int lane = threadIdx.x % kWave;
         for (int r = threadIdx.x / kWave; r < k; r += kBW) {
-            const double *row = R + (long long)r * n;
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four loads in flight, fixed order
+            const T *row = R + (long long)r * n;
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // four loads in flight, fixed order
             int j = lane;
             for (; j + 3 * kWave < n; j += 4 * kWave) {
                 s0 += row[j] * x[j];
@@ -212,7 +215,7 @@ struct Ctx {
                 s3 += row[j + 3 * kWave] * x[j + 3 * kWave];
             }
             for (; j < n; j += kWave) s0 += row[j] * x[j];
-            const double s = wave_sum((s0 + s1) + (s2 + s3));
+            const double s = (double)wave_sum((s0 + s1) + (s2 + s3));
             if (lane == 0) rx[r] = s;
         }
         __syncthreads();
@@ -234,15 +237,15 @@ struct Ctx {
         }
         __syncthreads();
         for (int j = threadIdx.x; j < n; j += kBT) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four loads in flight, fixed order
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // four loads in flight, fixed order
             int r = 0;
             for (; r + 3 < k; r += 4) {
-                s0 += R[(long long)r * n + j] * rb[r];
-                s1 += R[(long long)(r + 1) * n + j] * rb[r + 1];
-                s2 += R[(long long)(r + 2) * n + j] * rb[r + 2];
-                s3 += R[(long long)(r + 3) * n + j] * rb[r + 3];
+                s0 += R[(long long)r * n + j] * (T)rb[r];
+                s1 += R[(long long)(r + 1) * n + j] * (T)rb[r + 1];
+                s2 += R[(long long)(r + 2) * n + j] * (T)rb[r + 2];
+                s3 += R[(long long)(r + 3) * n + j] * (T)rb[r + 3];
             }
-            for (; r < k; ++r) s0 += R[(long long)r * n + j] * rb[r];
+            for (; r < k; ++r) s0 += R[(long long)r * n + j] * (T)rb[r];
             qx[j] = qd[j] * x[j] + ((s0 + s1) + (s2 + s3));
         }
         __syncthreads();
@@ -250,8 +253,8 @@ struct Ctx {
 
     // Projection of the m-vector v onto the affine cones, scaling affd
     // (reference src/cone_dispatch.cu:38-57 on the row space).
-    __device__ __noinline__ void project_affine(double *v, double *warm) const {
-        const double *d = mv(wAFD);
+    __device__ __noinline__ void project_affine(T *v, double *warm) const {
+        const T *d = mv(wAFD);
         const int nshort = I.ncones - I.nlong;
         for (int c = threadIdx.x; c < nshort; c += kBT) {
             const Cone cn = cones[c];
@@ -259,32 +262,38 @@ struct Ctx {
             if (cn.type == CONE_STANDARD_SOC || cn.type == CONE_ROTATED_SOC) {
                 double pv[kMaxThreadCone + 2];
                 for (int q = 0; q < cn.vdim + 2; ++q) pv[q] = v[s + q];
-                soc_project_block(pv, cn.vdim, d + s, cn.type == CONE_ROTATED_SOC, warm + c);
-                for (int q = 0; q < cn.vdim + 2; ++q) v[s + q] = pv[q];
+                if constexpr (std::is_same<T, double>::value) {
+                    soc_project_block(pv, cn.vdim, d + s, cn.type == CONE_ROTATED_SOC, warm + c);
+                } else {
+                    double pd[kMaxThreadCone + 2];
+                    for (int q = 0; q < cn.vdim + 2; ++q) pd[q] = d[s + q];
+                    soc_project_block(pv, cn.vdim, pd, cn.type == CONE_ROTATED_SOC, warm + c);
+                }
+                for (int q = 0; q < cn.vdim + 2; ++q) v[s + q] = (T)pv[q];
             } else if (cn.type == CONE_EXPONENTIAL) {
                 double o[3];
                 exp_project_point(v[s], v[s + 1], v[s + 2], d[s], d[s + 1], d[s + 2], o);
-                v[s] = o[0];
-                v[s + 1] = o[1];
-                v[s + 2] = o[2];
+                v[s] = (T)o[0];
+                v[s + 1] = (T)o[1];
+                v[s + 2] = (T)o[2];
             } else {
                 double o[3];
                 const double d0 = d[s], d1 = d[s + 1], d2 = d[s + 2];
                 power_project_point(v[s] / d0, v[s + 1] / d1, v[s + 2] / d2, d0 * d0, d1 * d1, d2 * d2, cn.alpha, o);
-                v[s] = o[0] * d0;
-                v[s + 1] = o[1] * d1;
-                v[s + 2] = o[2] * d2;
+                v[s] = (T)(o[0] * d0);
+                v[s + 1] = (T)(o[1] * d1);
+                v[s + 2] = (T)(o[2] * d2);
             }
         }
         __syncthreads();
         // long second-order cones: the whole workgroup on one cone at a time
         for (int c = nshort; c < I.ncones; ++c) {
             const Cone cn = cones[c];
-            double *pv = v + cn.start;
-            const double *pd = d + cn.start;
-            auto P = [&](int m) { return pv[m]; };
-            auto E = [&](int m) { return pd[m]; };
-            auto Set = [&](int m, double val) { pv[m] = val; };
+            T *pv = v + cn.start;
+            const T *pd = d + cn.start;
+            auto P = [&](int m) { return (double)pv[m]; };
+            auto E = [&](int m) { return (double)pd[m]; };
+            auto Set = [&](int m, double val) { pv[m] = (T)val; };
             auto sum = [&](double x) { return red.sum(x); };
             if (cn.type == CONE_ROTATED_SOC) soc_project_coop<true>(P, E, Set, cn.vdim, warm + c, sum, false);
             else soc_project_coop<false>(P, E, Set, cn.vdim, warm + c, sum, false);
@@ -293,9 +302,19 @@ struct Ctx {
     }
 };
 
-// Loop state (identical in every thread).
+// A problem that leaves the FP32 phase of the mixed batch before
+// terminating (its iterates stopped improving at the FP32 rounding floor).
+constexpr int kStalled = 100;
+constexpr int kSatStall = 4;  // FP32 inner solves in a row at the inner limit
+
+// Loop state (identical in every thread).  best_err / best_iter track the
+// KKT error of the FP32 phase (a problem whose error has not halved for
+// max(1000, half its iterations) leaves that phase as kStalled), sat the
+// FP32 inner solves in a row that reached the inner iteration limit, and
+// eval_now asks for a KKT evaluation at the next pass.
 struct State {
-    int total, inner_count, inner_total, is_major, do_restart, reason;
+    int total, inner_count, inner_total, is_major, do_restart, reason, best_iter, sat, eval_now, pad;
+    double best_err;
     double pw, pw_err_sum, pw_last_err, best_pw, best_gap;
     double fpe, init_fpe, last_trial_fpe, inner_tol, cached_inv_tau, tol_scale;
     double abs_pr, abs_du, rel_pr, rel_du, pobj, dobj, gap, rel_gap;
@@ -303,12 +322,12 @@ struct State {
 };
 
 // Projected BB inner solve of the primal prox (reference src/pdhg_core_op.cu:495-669).
-__device__ __noinline__ void bb_prox(const Ctx &C, State &S, double tau) {
+template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &S, double tau) {
     const Params &P = *C.P;
     const int n = C.I.n;
-    double *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *grad = C.nv(vGR), *dirn = C.nv(vDIR), *qx = C.nv(vQX),
+    T *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *grad = C.nv(vGR), *dirn = C.nv(vDIR), *qx = C.nv(vQX),
            *pb = C.nv(vPB), *md = C.nv(vMD), *mi = C.nv(vMI);
-    const double *c = C.nv(vC), *lb = C.nv(vLB), *ub = C.nv(vUB), *aty = C.nv(vATY), *dh = C.nv(vDH);
+    const T *c = C.nv(vC), *lb = C.nv(vLB), *ub = C.nv(vUB), *aty = C.nv(vATY), *dh = C.nv(vDH);
     const double inv = 1.0 / tau;
     double a0 = 1.0 / inv;
     const bool pre = P.precond != 0;
@@ -370,10 +389,11 @@ __device__ __noinline__ void bb_prox(const Ctx &C, State &S, double tau) {
     for (int i = threadIdx.x; i < n; i += kBT) xr[i] = 2.0 * xp[i] - x[i];
     __syncthreads();
     S.inner_total += it - 1;
+    if (!std::is_same<T, double>::value) S.sat = (P.inner_limit > 1 && it >= P.inner_limit) ? S.sat + 1 : 0;
 }
 
 // One PDHG step (reference src/pdhg_core_op.cu:705-830).
-__device__ __noinline__ void pdhg_step(const Ctx &C, State &S) {
+template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     double tau = I.step / S.pw;
@@ -382,8 +402,8 @@ __device__ __noinline__ void pdhg_step(const Ctx &C, State &S) {
         tau /= 100;
     }
     const double sigma = I.step * S.pw;
-    double *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *aty = C.nv(vATY), *ds = C.nv(vDS);
-    const double *c = C.nv(vC), *lb = C.nv(vLB), *ub = C.nv(vUB), *qd = C.nv(vQD);
+    T *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *aty = C.nv(vATY), *ds = C.nv(vDS);
+    const T *c = C.nv(vC), *lb = C.nv(vLB), *ub = C.nv(vUB), *qd = C.nv(vQD);
     C.At(C.mv(wY), aty);
     const int tc = S.total;
     const bool store = S.is_major || ((tc + 2) % print_frequency(tc + 2)) == 0;
@@ -412,8 +432,8 @@ __device__ __noinline__ void pdhg_step(const Ctx &C, State &S) {
         bb_prox(C, S, tau);
     }
     S.inner_total++;
-    double *ax = C.mv(wAX), *y = C.mv(wY), *yp = C.mv(wYP), *yr = C.mv(wYR), *apr = C.mv(wAPR);
-    const double *g = C.mv(wG), *cl = C.mv(wCL), *cu = C.mv(wCU);
+    T *ax = C.mv(wAX), *y = C.mv(wY), *yp = C.mv(wYP), *yr = C.mv(wYR), *apr = C.mv(wAPR);
+    const T *g = C.mv(wG), *cl = C.mv(wCL), *cu = C.mv(wCU);
     C.A(xr, ax);
     if (m == 0) return;
     if (I.has_affine) {  // reference src/pdhg_core_op.cu:809-829
@@ -433,11 +453,11 @@ __device__ __noinline__ void pdhg_step(const Ctx &C, State &S) {
 }
 
 // reference src/pdhg_core_op.cu:832-845
-__device__ inline void halpern(const Ctx &C, const State &S) {
+template <class T> __device__ inline void halpern(const Ctx<T> &C, const State &S) {
     const double rho = C.P->rho;
     const double wgt = (double)(S.inner_count + 1) / (S.inner_count + 2);
-    double *x = C.nv(vX), *y = C.mv(wY);
-    const double *xr = C.nv(vXR), *x0 = C.nv(vX0), *yr = C.mv(wYR), *y0 = C.mv(wY0);
+    T *x = C.nv(vX), *y = C.mv(wY);
+    const T *xr = C.nv(vXR), *x0 = C.nv(vX0), *yr = C.mv(wYR), *y0 = C.mv(wY0);
     for (int i = threadIdx.x; i < C.I.n; i += kBT) {
         const double r = rho * xr[i] + (1.0 - rho) * x[i];
         x[i] = wgt * r + (1.0 - wgt) * x0[i];
@@ -450,10 +470,10 @@ __device__ inline void halpern(const Ctx &C, const State &S) {
 }
 
 // reference src/pdhg_core_op.cu:973-1035
-__device__ __noinline__ void fixed_point_error(const Ctx &C, State &S) {
+template <class T> __device__ __noinline__ void fixed_point_error(const Ctx<T> &C, State &S) {
     const int n = C.I.n, m = C.I.m;
-    double *dx = C.nv(vDX), *dy = C.mv(wDY), *aty = C.nv(vATY);
-    const double *xr = C.nv(vXR), *x = C.nv(vX), *yr = C.mv(wYR), *y = C.mv(wY);
+    T *dx = C.nv(vDX), *dy = C.mv(wDY), *aty = C.nv(vATY);
+    const T *xr = C.nv(vXR), *x = C.nv(vX), *yr = C.mv(wYR), *y = C.mv(wY);
     for (int i = threadIdx.x; i < n; i += kBT) dx[i] = xr[i] - x[i];
     for (int i = threadIdx.x; i < m; i += kBT) dy[i] = yr[i] - y[i];
     __syncthreads();
@@ -468,11 +488,11 @@ __device__ __noinline__ void fixed_point_error(const Ctx &C, State &S) {
 }
 
 // reference src/pdhg_core_op.cu:859-931
-__device__ __noinline__ void restart(const Ctx &C, State &S) {
+template <class T> __device__ __noinline__ void restart(const Ctx<T> &C, State &S) {
     const Params &P = *C.P;
     const int n = C.I.n, m = C.I.m;
-    double *x0 = C.nv(vX0), *x = C.nv(vX), *y0 = C.mv(wY0), *y = C.mv(wY);
-    const double *xp = C.nv(vXP), *yp = C.mv(wYP);
+    T *x0 = C.nv(vX0), *x = C.nv(vX), *y0 = C.mv(wY0), *y = C.mv(wY);
+    const T *xp = C.nv(vXP), *yp = C.mv(wYP);
     const double pd = sqrt(C.red.dot(n, [&](int i) { return (xp[i] - x0[i]) * (xp[i] - x0[i]); }));
     const double dd = sqrt(C.red.dot(m, [&](int i) { return (yp[i] - y0[i]) * (yp[i] - y0[i]); }));
     const double ratio = S.rel_du / S.rel_pr;
@@ -501,7 +521,7 @@ __device__ __noinline__ void restart(const Ctx &C, State &S) {
 }
 
 // reference src/utils.cu:291-322
-__device__ inline bool should_restart(const Ctx &C, State &S) {
+template <class T> __device__ inline bool should_restart(const Ctx<T> &C, State &S) {
     const Params &P = *C.P;
     bool r = false;
     if (S.total == P.ef) r = true;
@@ -515,14 +535,14 @@ __device__ inline bool should_restart(const Ctx &C, State &S) {
 }
 
 // KKT residuals (reference src/pdhg_core_op.cu:1037-1294).
-__device__ __noinline__ void residual(const Ctx &C, State &S) {
+template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     const bool linf = C.P->norm_inf != 0;
     const double tau = I.step / S.pw;
-    double *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX), *ds = C.nv(vDS), *pres = C.mv(wPRES),
+    T *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX), *ds = C.nv(vDS), *pres = C.mv(wPRES),
            *psl = C.mv(wPSL), *amem = C.mv(wAMEM);
-    const double *xp = C.nv(vXP), *yp = C.mv(wYP), *g = C.mv(wG), *cl = C.mv(wCL), *cu = C.mv(wCU),
+    const T *xp = C.nv(vXP), *yp = C.mv(wYP), *g = C.mv(wG), *cl = C.mv(wCL), *cu = C.mv(wCU),
                  *clf = C.mv(wCLF), *cuf = C.mv(wCUF), *cr = C.mv(wCR), *c = C.nv(vC), *lb = C.nv(vLB),
                  *ub = C.nv(vUB), *vr = C.nv(vVR), *afd = C.mv(wAFD);
     C.A(xp, ax);
@@ -606,11 +626,11 @@ __device__ __noinline__ void residual(const Ctx &C, State &S) {
 }
 
 // Infeasibility rays (reference src/pdhg_core_op.cu:1296-1429).
-__device__ __noinline__ void infeasibility(const Ctx &C, State &S) {
+template <class T> __device__ __noinline__ void infeasibility(const Ctx<T> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
-    double *dx = C.nv(vDX), *dy = C.mv(wDY), *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX);
-    const double *lb = C.nv(vLB), *ub = C.nv(vUB), *cl = C.mv(wCL), *cu = C.mv(wCU), *c = C.nv(vC),
+    T *dx = C.nv(vDX), *dy = C.mv(wDY), *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX);
+    const T *lb = C.nv(vLB), *ub = C.nv(vUB), *cl = C.mv(wCL), *cu = C.mv(wCU), *c = C.nv(vC),
                  *clf = C.mv(wCLF), *cuf = C.mv(wCUF), *g = C.mv(wG), *lbf = C.nv(vLBF), *ubf = C.nv(vUBF),
                  *cr = C.mv(wCR), *vr = C.nv(vVR);
     for (int j = threadIdx.x; j < n; j += kBT) {
@@ -662,7 +682,7 @@ __device__ __noinline__ void infeasibility(const Ctx &C, State &S) {
 }
 
 // reference src/utils.cu:219-289 (the batch applies no wall-clock limit)
-__device__ inline void check_termination(const Ctx &C, State &S) {
+template <class T> __device__ inline void check_termination(const Ctx<T> &C, State &S) {
     const Params &P = *C.P;
     if (S.rel_du < P.eps_feas && S.rel_pr < P.eps_feas && S.rel_gap < P.eps_opt) {
         S.reason = TERMINATION_REASON_OPTIMAL;
@@ -687,13 +707,22 @@ __device__ inline void check_termination(const Ctx &C, State &S) {
 // passes of the main loop from its saved state (every loop is bounded), and
 // a problem that has terminated leaves A'y and Q x of its final point in the
 // slab for the reduced costs and returns at once on later launches.
+//
+// T = float is the FP32 phase of the mixed batch: iterates, matrix values and
+// products in FP32 (vpool, slabs), D, the k-vectors and the cone values in
+// FP64 (dpool, wslabs), and a problem whose KKT error stops improving leaves
+// with kStalled; the host then widens the iterates into the FP64 slabs and
+// the FP64 kernel re-evaluates every problem before it terminates.
+template <class T>
 __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const Inst *insts, const int *ipool,
-                                                          const double *vpool, double *slabs, const Cone *cones,
-                                                          State *states, Out *out, int budget) {
+                                                          const T *vpool, const double *dpool, T *slabs,
+                                                          double *wslabs, const Cone *cones, State *states, Out *out,
+                                                          int budget) {
     __shared__ double lds[2 * kBW];
     State S = states[blockIdx.x];
     if (S.reason != TERMINATION_REASON_UNSPECIFIED) return;
-    Ctx C;
+    constexpr bool fp32 = !std::is_same<T, double>::value;
+    Ctx<T> C;
     C.P = P;
     C.I = insts[blockIdx.x];
     C.arp = ipool + C.I.a_rp;
@@ -703,8 +732,9 @@ __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const
     C.av = vpool + C.I.a_v;
     C.atv = vpool + C.I.at_v;
     C.R = vpool + C.I.rdense;
-    C.D = vpool + C.I.dmat;
+    C.D = dpool + C.I.dmat;
     C.V = slabs + C.I.vec;
+    C.W = wslabs + C.I.wvec;
     C.cones = cones + C.I.cone0;
     C.red.lds = lds;
     const int ef = P->ef;
@@ -715,11 +745,26 @@ __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const
             break;
         }
         const int tc = S.total;
-        if (S.is_major || tc == 0 || tc % print_frequency(tc) == 0) {
+        if (S.is_major || tc == 0 || S.eval_now || tc % print_frequency(tc) == 0) {
+            S.eval_now = 0;
             residual(C, S);
             if (!C.I.has_affine && S.is_major && tc < 3 * ef) infeasibility(C, S);
             check_termination(C, S);
             if (S.reason != TERMINATION_REASON_UNSPECIFIED) break;
+            if (fp32) {  // FP32 floor: hand the problem to the FP64 kernel
+                const double err = fmax(S.rel_pr, fmax(S.rel_du, S.rel_gap));
+                if (err < 0.5 * S.best_err) {
+                    S.best_err = err;
+                    S.best_iter = tc;
+                } else if (tc - S.best_iter >= max(P->stall_iters, tc / 2)) {
+                    S.reason = kStalled;
+                    break;
+                }
+            }
+        }
+        if (fp32 && S.sat >= kSatStall) {
+            S.reason = kStalled;
+            break;
         }
         if (S.is_major || tc == 0) {
             S.do_restart = should_restart(C, S);
@@ -738,7 +783,7 @@ __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const
         S.inner_count++;
         S.total++;
     }
-    if (S.reason != TERMINATION_REASON_UNSPECIFIED) {
+    if (S.reason != TERMINATION_REASON_UNSPECIFIED && !fp32) {
         C.At(C.mv(wYP), C.nv(vATY));
         C.obj_product(C.nv(vXP), C.nv(vQX));
     }

@@ -13,7 +13,7 @@ import pytest
 import scipy.sparse as sp
 
 import pdhcg_amd
-from pdhcg_amd import QPData, workloads
+from pdhcg_amd import PRECISION_MIXED, QPData, workloads
 from tests.kkt import kkt
 from tests.oracle_lib import oracle_solve
 
@@ -98,3 +98,29 @@ def test_batch_portfolios_with_long_cones_match_oracle():
         _agree(p, g, o, affine=True)
         te = np.linalg.norm(np.sqrt(p.D) * (p.R @ g.x))
         assert te <= p.g[-1] * (1 + 1e-4)
+
+
+def test_mixed_batch_matches_oracle():
+    # FP32 iterates first, then every problem evaluated (and if need be
+    # finished) in FP64: statuses and residuals are FP64 ones, to the same bar
+    cases = [("lp", 40, 0), ("diag", 77, 0), ("lowrank", 80, 1), ("lowrank_diag", 151, 0), ("lowrank_dense_d", 70, 0),
+             ("lp", 150, 3), ("lowrank_diag", 200, 2)]
+    problems = [workloads.random_problem(kind, seed=seed, n=n, m=max(10, n // 3)) for kind, n, seed in cases]
+    problems += workloads.portfolio_batch(3, seed=12, n_range=(50, 300), k_range=(10, 40))
+    sols = pdhcg_amd.solve_batch(problems, precision=PRECISION_MIXED, eps=EPS, iteration_limit=200000,
+                                 time_limit=60.0)
+    for i, (p, g) in enumerate(zip(problems, sols)):
+        o = oracle_solve(p, eps=EPS, iteration_limit=200000)
+        _agree(p, g, o, affine=i >= len(cases))
+        # the reported residuals are the FP64 ones of the returned point
+        assert g.rel_primal < EPS and g.rel_dual < EPS and g.rel_gap < EPS
+
+
+def test_mixed_batch_limits_and_infeasibility():
+    infeasible = QPData(c=[1.0, 1.0], A=sp.csr_matrix([[1.0, 1.0]]), con_lb=[3.0], con_ub=[np.inf],
+                        var_lb=[0.0, 0.0], var_ub=[1.0, 1.0])
+    slow = workloads.random_problem("lowrank_diag", seed=5, n=80, m=30)
+    sols = pdhcg_amd.solve_batch([infeasible, slow], precision=PRECISION_MIXED, iteration_limit=20000)
+    assert sols[0].status == "PRIMAL_INFEASIBLE"
+    capped = pdhcg_amd.solve_batch([slow], precision=PRECISION_MIXED, eps=1e-12, iteration_limit=57)[0]
+    assert capped.status == "ITERATION_LIMIT" and capped.iterations == 57
