@@ -211,6 +211,20 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
         status[k] = status.get(k, 0) + 1
     iters = [s_.iterations for s_ in sols if s_ is not None]
     log(f"[bench] batch: solved in {batch_s:.2f}s, {status}")
+    # algorithmic bytes of the FP64 batch kernel over the whole solve: per
+    # outer iteration the two products with [A; F] (values and column
+    # indices 12 bytes per entry, row pointers) and about twelve FP64 vector
+    # passes over n + m; per Q product (one per outer iteration and one per
+    # BB body: the inner count) the R x and R'u passes over the dense k x n R
+    alg = 0.0
+    for p_, s_ in zip(problems, sols):
+        if s_ is None:
+            continue
+        n_, m_ = p_.n, p_.A.shape[0] + (p_.F.shape[0] if p_.F is not None else 0)
+        nnz_ = p_.A.nnz + (p_.F.nnz if p_.F is not None else 0)
+        k_ = p_.R.shape[0] if p_.R is not None else 0
+        alg += s_.iterations * (2 * (12.0 * nnz_ + 4.0 * (n_ + m_ + 2)) + 96.0 * (n_ + m_))
+        alg += s_.inner_iterations * 16.0 * k_ * n_
     sample = problems[:: max(1, count // 8)][:8]
     t = time.perf_counter()
     single = [pdhcg_amd.solve(p_, eps=eps, iteration_limit=100000) for p_ in sample]
@@ -222,6 +236,7 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
            "status": status, "iterations_mean": float(np.mean(iters)), "iterations_max": int(max(iters)),
            "single_device_ms_per_problem": single_ms, "single_sample": len(sample),
            "speedup_vs_single_device": single_ms / (1e3 * batch_s / count),
+           "algorithmic_bytes": alg, "algorithmic_GBps_end_to_end": alg / batch_s / 1e9,
            "note": "time_to_tol_s covers host scaling and packing plus the device solve of the whole batch"}
     # the mixed form: FP32 iterates and matrix values, then every problem
     # evaluated (and if need be finished) in FP64
@@ -476,6 +491,8 @@ def main():
     ap.add_argument("--warm-test", type=int, default=256, metavar="N")
     ap.add_argument("--conic-seconds", type=float, default=40.0,
                     help="time limit of each conic-mix solve (0: skip the conic mix)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched portfolio section (prints {\"batched\": ...})")
     ap.add_argument("--conic-only", action="store_true",
                     help="only run the conic-mix section (prints {\"conic_mix\": ...})")
     ap.add_argument("--warm-large-train", type=int, default=0, metavar="N",
@@ -505,6 +522,9 @@ def main():
     if args.conic_only:
         print(json.dumps({"conic_mix": run_conic(args.conic_seconds, args.eps, args.cpu_seconds, not args.no_cpu)}),
               flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": run_batch(args.batch, args.eps, args.cpu_seconds, not args.no_cpu)}), flush=True)
         return
 
     from pdhcg_amd import workloads

@@ -807,9 +807,14 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
 // src/kernels/pdhcg_psd_cone_kernels.cu:263-486).
 
 constexpr int kPsdMaxSweeps = 60;
-constexpr int kPsdThreads = 512;
-constexpr int kPsdChunk = 4;    // 2x2 block updates a thread loads before it stores (phase 2)
-constexpr int kPsdChunkV = 8;   // V' row-pair updates a thread loads before it stores
+// Workgroup sizes of the projection (one instance each) and, per size, the
+// 2x2 block updates and V' row-pair updates a thread loads before it stores
+// any (phase 2); the larger workgroup hides more latency per round and fits
+// 128 VGPRs with the smaller chunks.
+constexpr int kPsdThreads = 512, kPsdThreadsLarge = 1024;
+template <int NT> struct PsdChunks {
+    static constexpr int a = NT >= 1024 ? 2 : 4, v = NT >= 1024 ? 4 : 8;
+};
 
 // One PSD block of a launch.
 struct PsdBlock {
@@ -886,12 +891,13 @@ __device__ __forceinline__ void psd_tile_xy(const double *X, const double *Y, in
     }
 }
 
-template <class X>
-__global__ void __launch_bounds__(kPsdThreads) psd_project_kernel(X *v, const PsdBlock *blocks, double *work,
+template <class X, int NT>
+__global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *blocks, double *work,
                                                                    int site, const int *gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double psd_lds[];
-    __shared__ double red[2 * (kPsdThreads / kWave)];
+    constexpr int kPsdChunk = PsdChunks<NT>::a, kPsdChunkV = PsdChunks<NT>::v;
+    __shared__ double red[2 * (NT / kWave)];
     __shared__ int stop;
     const PsdBlock blk = blocks[blockIdx.x];
     const int start = blk.start, k = blk.order, mode = blk.mode;

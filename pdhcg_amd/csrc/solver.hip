@@ -146,15 +146,24 @@ inline size_t psd_lds_budget() {
 // eigenvector set warm-starts it (0: the iterate, 1: the dual residual).
 // Dynamic LDS above the default limit is requested once per instance.
 template <class X> void psd_launch(const PsdSet &g, X *v, int site, const int *gate, hipStream_t st) {
-    static bool raised = false;
+    static bool raised[2] = {false, false};
     if (!g.count) return;
-    if (!raised && g.lds > 0) {
-        HIP_OK(hipFuncSetAttribute((const void *)psd_project_kernel<X>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)psd_lds_budget()));
-        raised = true;
-    }
-    hipLaunchKernelGGL(psd_project_kernel<X>, dim3(g.count), dim3(kPsdThreads), g.lds, st, v, (const PsdBlock *)g.blocks,
-                       g.work, site, gate);
+    // PDHCG_PSD_THREADS=512|1024 picks the workgroup size (default 1024)
+    static const int threads = [] {
+        const char *e = std::getenv("PDHCG_PSD_THREADS");
+        return e && std::atoi(e) == 512 ? kPsdThreads : kPsdThreadsLarge;
+    }();
+    auto go = [&](auto kernel, int which, int nt) {
+        if (!raised[which] && g.lds > 0) {
+            HIP_OK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)psd_lds_budget()));
+            raised[which] = true;
+        }
+        hipLaunchKernelGGL(kernel, dim3(g.count), dim3(nt), g.lds, st, v, (const PsdBlock *)g.blocks, g.work, site,
+                           gate);
+    };
+    if (threads == kPsdThreadsLarge) go(psd_project_kernel<X, kPsdThreadsLarge>, 1, kPsdThreadsLarge);
+    else go(psd_project_kernel<X, kPsdThreads>, 0, kPsdThreads);
 }
 
 class DevConeSet {
