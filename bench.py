@@ -234,6 +234,7 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
                            count, sum(p_.n for p_ in problems)),
            "problems": count, "time_to_tol_s": batch_s, "ms_per_problem": 1e3 * batch_s / count,
            "status": status, "iterations_mean": float(np.mean(iters)), "iterations_max": int(max(iters)),
+           "inner_iterations_mean": float(np.mean([s_.inner_iterations for s_ in sols if s_ is not None])),
            "single_device_ms_per_problem": single_ms, "single_sample": len(sample),
            "speedup_vs_single_device": single_ms / (1e3 * batch_s / count),
            "algorithmic_bytes": alg, "algorithmic_GBps_end_to_end": alg / batch_s / 1e9,
@@ -252,6 +253,7 @@ def run_batch(count, eps, cpu_seconds, with_cpu):
     rel = [abs(a.pobj - b.pobj) / (1.0 + abs(b.pobj)) for a, b in zip(msols, sols) if a is not None and b is not None]
     out["mixed"] = {"time_to_tol_s": mixed_s, "ms_per_problem": 1e3 * mixed_s / count, "status": mstatus,
                     "iterations_mean": float(np.mean(miters)), "iterations_max": int(max(miters)),
+                    "inner_iterations_mean": float(np.mean([s_.inner_iterations for s_ in msols if s_ is not None])),
                     "max_rel_objective_diff_vs_fp64": float(max(rel)) if rel else None,
                     "speedup_vs_fp64_batch": batch_s / mixed_s,
                     "speedup_vs_single_device": single_ms / (1e3 * mixed_s / count)}
@@ -397,13 +399,13 @@ def run_warm(train_count, test_count, eps, n=2000, m=1000, nnz=20000, singles=4,
            "train_mfma_frac": tflops / 2500.0, "predict_s": predict_s}
     for tol in (tols or (eps, 1e-4)):
         t = time.perf_counter()
-        cold = pdhcg_amd.solve_batch(pte, eps=tol, iteration_limit=100000, time_limit=300.0)
+        cold = pdhcg_amd.solve_batch(pte, eps=tol, iteration_limit=100000, time_limit=300.0, verbose=1)
         cold_s = time.perf_counter() - t
         t = time.perf_counter()
-        warm = pdhcg_amd.solve_batch(warm_p, eps=tol, iteration_limit=100000, time_limit=300.0)
+        warm = pdhcg_amd.solve_batch(warm_p, eps=tol, iteration_limit=100000, time_limit=300.0, verbose=1)
         warm_s = time.perf_counter() - t
         t = time.perf_counter()
-        primal = pdhcg_amd.solve_batch(primal_p, eps=tol, iteration_limit=100000, time_limit=300.0)
+        primal = pdhcg_amd.solve_batch(primal_p, eps=tol, iteration_limit=100000, time_limit=300.0, verbose=1)
         primal_s = time.perf_counter() - t
         if guard:
             t = time.perf_counter()

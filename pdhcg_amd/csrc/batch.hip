@@ -468,6 +468,11 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             const char *e = std::getenv("PDHCG_MIXED_STALL");
             dp.stall_iters = e ? std::atoi(e) : std::max(1000, 5 * dp.ef);
         }
+        // the FP32 phase hands a problem over once its KKT error is within a
+        // decade of the tolerance: FP32 iterates take the bulk of the
+        // iterations, FP64 ones the last decade (many small problems have
+        // their FP32 floor there, and iterating on at the floor is waste)
+        dp.handover = 10.0 * std::fmax(dp.eps_opt, dp.eps_feas);
         const bool mixed = pdhcg_get_precision() == PDHCG_PRECISION_MIXED;
         auto track = [&](void *p) {
             dev.push_back(p);
@@ -539,14 +544,79 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                 }
             }
         };
+        // launches of the main phase's kernel at either workgroup size
+        auto launch32 = [&](int bt, int budget) {
+            if (bt == kBTSmall)
+                hipLaunchKernelGGL((batch_solve_kernel<float, kBTSmall>), dim3(B), dim3(kBTSmall), 0, st, d_params,
+                                   d_insts, d_ipool, (const float *)d_vpool32, (const double *)d_vpool, d_slabs32,
+                                   d_wslabs, d_cones, d_states, d_out, budget);
+            else
+                hipLaunchKernelGGL((batch_solve_kernel<float, kBT>), dim3(B), dim3(kBT), 0, st, d_params, d_insts,
+                                   d_ipool, (const float *)d_vpool32, (const double *)d_vpool, d_slabs32, d_wslabs,
+                                   d_cones, d_states, d_out, budget);
+        };
+        auto launch64 = [&](int bt, int budget) {
+            if (bt == kBTSmall)
+                hipLaunchKernelGGL((batch_solve_kernel<double, kBTSmall>), dim3(B), dim3(kBTSmall), 0, st, d_params,
+                                   d_insts, d_ipool, (const double *)d_vpool, (const double *)d_vpool, d_slabs,
+                                   d_wslabs, d_cones, d_states, d_out, budget);
+            else
+                hipLaunchKernelGGL((batch_solve_kernel<double, kBT>), dim3(B), dim3(kBT), 0, st, d_params, d_insts,
+                                   d_ipool, (const double *)d_vpool, (const double *)d_vpool, d_slabs, d_wslabs,
+                                   d_cones, d_states, d_out, budget);
+        };
+        // Threads per problem (the autotuned launch shape): both sizes run
+        // the same short probe from the same saved state (loop states, slabs
+        // and FP64 slabs restored after each), and the faster one runs the
+        // batch.  PDHCG_BATCH_THREADS=256|512 skips the probe.
+        int bt = kBT;
+        if (const char *e = std::getenv("PDHCG_BATCH_THREADS")) {
+            bt = std::atoi(e) == kBTSmall ? kBTSmall : kBT;
+        } else if (window > 1) {
+            const int probe = std::min(window, 25);
+            const size_t sbytes = (size_t)soff[B] * (mixed ? sizeof(float) : sizeof(double));
+            void *slab_src = mixed ? (void *)d_slabs32 : (void *)d_slabs;
+            void *s_copy = nullptr, *w_copy = nullptr, *st_copy = nullptr;
+            HIP_OK(hipMalloc(&s_copy, std::max<size_t>(sbytes, 1)));
+            dev.push_back(s_copy);
+            HIP_OK(hipMalloc(&w_copy, std::max<size_t>((size_t)woff[B], 1) * sizeof(double)));
+            dev.push_back(w_copy);
+            HIP_OK(hipMalloc(&st_copy, (size_t)B * sizeof(State)));
+            dev.push_back(st_copy);
+            auto copy_all = [&](bool save) {
+                auto cp = [&](void *a, void *b, size_t n) {
+                    if (n) HIP_OK(hipMemcpyAsync(save ? b : a, save ? a : b, n, hipMemcpyDeviceToDevice, st));
+                };
+                cp(slab_src, s_copy, sbytes);
+                cp(d_wslabs, w_copy, (size_t)woff[B] * sizeof(double));
+                cp(d_states, st_copy, (size_t)B * sizeof(State));
+            };
+            hipEvent_t e0, e1;
+            HIP_OK(hipEventCreate(&e0));
+            HIP_OK(hipEventCreate(&e1));
+            float ms[2] = {0.0f, 0.0f};
+            const int sizes[2] = {kBT, kBTSmall};
+            copy_all(true);
+            for (int c = 0; c < 2; ++c) {
+                HIP_OK(hipEventRecord(e0, st));
+                if (mixed) launch32(sizes[c], probe);
+                else launch64(sizes[c], probe);
+                HIP_OK(hipEventRecord(e1, st));
+                HIP_OK(hipEventSynchronize(e1));
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipEventElapsedTime(&ms[c], e0, e1));
+                copy_all(false);
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            HIP_OK(hipStreamSynchronize(st));
+            bt = ms[1] < 0.95f * ms[0] ? kBTSmall : kBT;
+            if (P.verbose >= 1)
+                std::fprintf(stderr, "[batch] threads per problem: %d (probe of %d passes: 512 -> %.1f ms, 256 -> %.1f ms)\n",
+                             bt, probe, ms[0], ms[1]);
+        }
         if (mixed) {
-            run_phase(
-                [&] {
-                    hipLaunchKernelGGL(batch_solve_kernel<float>, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool,
-                                       (const float *)d_vpool32, (const double *)d_vpool, d_slabs32, d_wslabs, d_cones,
-                                       d_states, d_out, window);
-                },
-                "FP32", INT_MAX);
+            run_phase([&] { launch32(bt, window); }, "FP32", INT_MAX);
             // FP32 -> FP64: the iterates widened exactly into the FP64 slabs
             // (the FP64 problem data stay), and every problem that has not
             // hit a limit evaluated again in FP64 before it may terminate
@@ -565,13 +635,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         // evaluates every problem in FP64
         const bool stop = time_up || cancelled;
         const bool was_time_up = time_up, was_cancelled = cancelled;
-        run_phase(
-            [&] {
-                hipLaunchKernelGGL(batch_solve_kernel<double>, dim3(B), dim3(kBT), 0, st, d_params, d_insts, d_ipool,
-                                   (const double *)d_vpool, (const double *)d_vpool, d_slabs, d_wslabs, d_cones,
-                                   d_states, d_out, stop ? 1 : window);
-            },
-            "FP64", stop ? 1 : INT_MAX);
+        run_phase([&] { launch64(bt, stop ? 1 : window); }, "FP64", stop ? 1 : INT_MAX);
         if (stop) {
             time_up = was_time_up;
             cancelled = was_cancelled;

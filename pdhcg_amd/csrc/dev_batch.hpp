@@ -29,8 +29,11 @@
 namespace pdhcg {
 namespace batch {
 
-constexpr int kBT = 512;  // threads per problem (two waves per SIMD at 248 VGPRs)
-constexpr int kBW = kBT / kWave;
+// Threads per problem: the launch picks 512 (two waves per SIMD at 248
+// VGPRs) or 256 by timing (batch.hip); device loops stride by the template
+// parameter BT, a compile-time constant.
+constexpr int kBT = 512, kBTSmall = 256;
+constexpr int kBW = kBT / kWave;  // waves of the largest workgroup (reduction slots)
 constexpr int kMaxRank = 256;  // low-rank terms up to this rank (R x, u live in the slab)
 
 enum QKind { kQNone = 0, kQDiag = 1, kQLowRank = 2 };
@@ -38,6 +41,7 @@ enum QKind { kQNone = 0, kQDiag = 1, kQLowRank = 2 };
 // Loop parameters shared by the batch (pdhg_parameters_t fields).
 struct Params {
     int ef, iteration_limit, norm_inf, inner_limit, precond, stall_iters;
+    double handover;  // FP32 phase: KKT error below which the FP64 kernel takes over
     double eps_opt, eps_feas, eps_inf, rho;
     double inner_tol0, inner_min_tol;
     double i_smooth, k_p, k_i, k_d, suff, nec, art;
@@ -92,7 +96,7 @@ __device__ inline int print_frequency(int it) {
 __device__ inline double clampd(double v, double lo, double hi) { return fmax(lo, fmin(v, hi)); }
 
 // Workgroup-wide reductions; every thread receives the result (fixed order).
-struct Red {
+template <int BT> struct Red {
     double *lds;  // 2 * kBW doubles
     __device__ double sum(double v) const {
         v = wave_sum(v);
@@ -100,7 +104,7 @@ struct Red {
         if (lane == 0) lds[w] = v;
         __syncthreads();
         double s = lds[0];
-        for (int i = 1; i < kBW; ++i) s += lds[i];
+        for (int i = 1; i < BT / kWave; ++i) s += lds[i];
         __syncthreads();
         return s;
     }
@@ -110,14 +114,14 @@ struct Red {
         if (lane == 0) lds[w] = v;
         __syncthreads();
         double s = lds[0];
-        for (int i = 1; i < kBW; ++i) s = fmax(s, lds[i]);
+        for (int i = 1; i < BT / kWave; ++i) s = fmax(s, lds[i]);
         __syncthreads();
         return s;
     }
     // L2 or max norm of the values f(i), i < count
     template <class F> __device__ double norm(int count, bool inf, F f) const {
         double a = 0.0;
-        for (int i = threadIdx.x; i < count; i += kBT) {
+        for (int i = threadIdx.x; i < count; i += BT) {
             const double v = f(i);
             a = inf ? fmax(a, fabs(v)) : a + v * v;
         }
@@ -125,7 +129,7 @@ struct Red {
     }
     template <class F> __device__ double dot(int count, F f) const {
         double a = 0.0;
-        for (int i = threadIdx.x; i < count; i += kBT) a += f(i);
+        for (int i = threadIdx.x; i < count; i += BT) a += f(i);
         return sum(a);
     }
 };
@@ -134,12 +138,12 @@ struct Red {
 // and two waves per SIMD there is little occupancy to hide memory latency,
 // so each lane keeps four independent partial sums (four loads in flight)
 // and adds them in a fixed order.
-template <class T>
+template <class T, int BT>
 __device__ inline void spmv(const int *rp, const int *ci, const T *v, const T *x, T *y, int rows,
                             int wave_rows) {
     if (wave_rows) {
         const int lane = threadIdx.x % kWave;
-        for (int r = threadIdx.x / kWave; r < rows; r += kBW) {
+        for (int r = threadIdx.x / kWave; r < rows; r += BT / kWave) {
             T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
             const int end = rp[r + 1];
             int e = rp[r] + lane;
@@ -154,7 +158,7 @@ __device__ inline void spmv(const int *rp, const int *ci, const T *v, const T *x
             if (lane == 0) y[r] = s;
         }
     } else {
-        for (int r = threadIdx.x; r < rows; r += kBT) {
+        for (int r = threadIdx.x; r < rows; r += BT) {
             T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
             const int end = rp[r + 1];
             int e = rp[r];
@@ -175,7 +179,7 @@ __device__ inline void spmv(const int *rp, const int *ci, const T *v, const T *x
 // storage and product type of the iterates and matrix values (double, or
 // float for the FP32 phase of the mixed batch); the k-vectors, per-cone
 // values, D and every reduction and scalar stay FP64.
-template <class T> struct Ctx {
+template <class T, int BT> struct Ctx {
     const Params *P;
     Inst I;
     const int *arp, *aci, *atrp, *atci;
@@ -184,13 +188,13 @@ template <class T> struct Ctx {
     T *V;       // n- and m-vectors of the slab
     double *W;  // k-vectors and per-cone values (FP64)
     const Cone *cones;
-    Red red;
+    Red<BT> red;
     __device__ T *nv(int which) const { return V + (long long)which * I.n; }
     __device__ T *mv(int which) const { return V + (long long)kNVec * I.n + (long long)which * I.m; }
     __device__ double *kv(int which) const { return W + (long long)which * I.k; }
     __device__ double *cv(int which) const { return W + (long long)kKVec * I.k + (long long)which * I.ncones; }
-    __device__ void A(const T *x, T *y) const { spmv(arp, aci, av, x, y, I.m, I.a_long); }
-    __device__ void At(const T *y, T *x) const { spmv(atrp, atci, atv, y, x, I.n, I.at_long); }
+    __device__ void A(const T *x, T *y) const { spmv<T, BT>(arp, aci, av, x, y, I.m, I.a_long); }
+    __device__ void At(const T *y, T *x) const { spmv<T, BT>(atrp, atci, atv, y, x, I.n, I.at_long); }
 
     // qx = Q x (reference src/pdhg_core_op.cu:305-413): diagonal part plus R'DR x
     __device__ __noinline__ void obj_product(const T *x, T *qx) const {
@@ -198,13 +202,13 @@ template <class T> struct Ctx {
         if (I.qkind == kQNone) return;
         const T *qd = nv(vQD);
         if (I.qkind == kQDiag) {
-            for (int i = threadIdx.x; i < n; i += kBT) qx[i] = qd[i] * x[i];
+            for (int i = threadIdx.x; i < n; i += BT) qx[i] = qd[i] * x[i];
             __syncthreads();
             return;
         }
         double *rx = kv(0), *rb = kv(1);
         const int lane = threadIdx.x % kWave;
-        for (int r = threadIdx.x / kWave; r < k; r += kBW) {
+        for (int r = threadIdx.x / kWave; r < k; r += BT / kWave) {
             const T *row = R + (long long)r * n;
             T s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // four loads in flight, fixed order
             int j = lane;
@@ -222,9 +226,9 @@ template <class T> struct Ctx {
         // u = D rx: diagonal, or the symmetric part of a dense D from its
         // upper triangle (reference src/pdhg_core_op.cu:272-303)
         if (I.dkind == 1) {
-            for (int r = threadIdx.x; r < k; r += kBT) rb[r] = rx[r] * D[r];
+            for (int r = threadIdx.x; r < k; r += BT) rb[r] = rx[r] * D[r];
         } else if (I.dkind == 2) {
-            for (int i = threadIdx.x; i < k; i += kBT) {
+            for (int i = threadIdx.x; i < k; i += BT) {
                 double a = 0.0;
                 for (int j = 0; j < k; ++j) {
                     const int lo = i < j ? i : j, hi = i < j ? j : i;
@@ -233,10 +237,10 @@ template <class T> struct Ctx {
                 rb[i] = a;
             }
         } else {
-            for (int r = threadIdx.x; r < k; r += kBT) rb[r] = rx[r];
+            for (int r = threadIdx.x; r < k; r += BT) rb[r] = rx[r];
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < n; j += kBT) {
+        for (int j = threadIdx.x; j < n; j += BT) {
             T s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // four loads in flight, fixed order
             int r = 0;
             for (; r + 3 < k; r += 4) {
@@ -256,7 +260,7 @@ template <class T> struct Ctx {
     __device__ __noinline__ void project_affine(T *v, double *warm) const {
         const T *d = mv(wAFD);
         const int nshort = I.ncones - I.nlong;
-        for (int c = threadIdx.x; c < nshort; c += kBT) {
+        for (int c = threadIdx.x; c < nshort; c += BT) {
             const Cone cn = cones[c];
             const int s = cn.start;
             if (cn.type == CONE_STANDARD_SOC || cn.type == CONE_ROTATED_SOC) {
@@ -305,7 +309,7 @@ template <class T> struct Ctx {
 // A problem that leaves the FP32 phase of the mixed batch before
 // terminating (its iterates stopped improving at the FP32 rounding floor).
 constexpr int kStalled = 100;
-constexpr int kSatStall = 4;  // FP32 inner solves in a row at the inner limit
+constexpr int kSatStall = 1;  // FP32 inner solves at the inner limit before the hand-over
 
 // Loop state (identical in every thread).  best_err / best_iter track the
 // KKT error of the FP32 phase (a problem whose error has not halved for
@@ -322,7 +326,7 @@ struct State {
 };
 
 // Projected BB inner solve of the primal prox (reference src/pdhg_core_op.cu:495-669).
-template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &S, double tau) {
+template <class T, int BT> __device__ __noinline__ void bb_prox(const Ctx<T, BT> &C, State &S, double tau) {
     const Params &P = *C.P;
     const int n = C.I.n;
     T *x = C.nv(vX), *xp = C.nv(vXP), *xr = C.nv(vXR), *grad = C.nv(vGR), *dirn = C.nv(vDIR), *qx = C.nv(vQX),
@@ -333,7 +337,7 @@ template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &
     const bool pre = P.precond != 0;
     if (pre && S.cached_inv_tau != inv) {
         double sacc = 0.0;
-        for (int i = threadIdx.x; i < n; i += kBT) {
+        for (int i = threadIdx.x; i < n; i += BT) {
             double mm = dh[i] + inv;
             if (mm <= 0.0) mm = 1.0;
             md[i] = mm;
@@ -346,7 +350,7 @@ template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &
     }
     if (pre) a0 *= S.tol_scale * S.tol_scale;
     C.obj_product(x, qx);
-    for (int i = threadIdx.x; i < n; i += kBT) {
+    for (int i = threadIdx.x; i < n; i += BT) {
         const double g = qx[i] + c[i] - aty[i];
         double nx = pre ? x[i] - a0 * mi[i] * g : x[i] - a0 * g;
         nx = clampd(nx, lb[i], ub[i]);
@@ -367,7 +371,7 @@ template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &
         if (nrm <= S.inner_tol) break;
         C.obj_product(xp, qx);
         double tacc = 0.0;
-        for (int i = threadIdx.x; i < n; i += kBT) {
+        for (int i = threadIdx.x; i < n; i += BT) {
             const double last = grad[i];
             const double cur = qx[i] + c[i] - aty[i] + inv * (xp[i] - x[i]);
             pb[i] = cur - last;
@@ -376,7 +380,7 @@ template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &
         }
         const double tmp = C.red.sum(tacc);
         const double alpha = pre ? stms / tmp : (nrm * nrm) / tmp;
-        for (int i = threadIdx.x; i < n; i += kBT) {
+        for (int i = threadIdx.x; i < n; i += BT) {
             const double cur = xp[i];
             double nx = pre ? cur - alpha * mi[i] * grad[i] : cur - alpha * grad[i];
             nx = clampd(nx, lb[i], ub[i]);
@@ -386,14 +390,14 @@ template <class T> __device__ __noinline__ void bb_prox(const Ctx<T> &C, State &
         __syncthreads();
         it++;
     }
-    for (int i = threadIdx.x; i < n; i += kBT) xr[i] = 2.0 * xp[i] - x[i];
+    for (int i = threadIdx.x; i < n; i += BT) xr[i] = 2.0 * xp[i] - x[i];
     __syncthreads();
     S.inner_total += it - 1;
     if (!std::is_same<T, double>::value) S.sat = (P.inner_limit > 1 && it >= P.inner_limit) ? S.sat + 1 : 0;
 }
 
 // One PDHG step (reference src/pdhg_core_op.cu:705-830).
-template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ __noinline__ void pdhg_step(const Ctx<T, BT> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     double tau = I.step / S.pw;
@@ -408,7 +412,7 @@ template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State
     const int tc = S.total;
     const bool store = S.is_major || ((tc + 2) % print_frequency(tc + 2)) == 0;
     if (I.qkind == kQNone) {
-        for (int i = threadIdx.x; i < n; i += kBT) {
+        for (int i = threadIdx.x; i < n; i += BT) {
             const double xi = x[i];
             const double t = xi - tau * (c[i] - aty[i]);
             const double pj = clampd(t, lb[i], ub[i]);
@@ -420,7 +424,7 @@ template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State
         }
         __syncthreads();
     } else if (I.qkind == kQDiag) {
-        for (int i = threadIdx.x; i < n; i += kBT) {
+        for (int i = threadIdx.x; i < n; i += BT) {
             const double xi = x[i];
             const double t = (xi - tau * (c[i] - aty[i])) / (1.0 + tau * qd[i]);
             const double pj = clampd(t, lb[i], ub[i]);
@@ -437,11 +441,11 @@ template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State
     C.A(xr, ax);
     if (m == 0) return;
     if (I.has_affine) {  // reference src/pdhg_core_op.cu:809-829
-        for (int i = threadIdx.x; i < m; i += kBT) apr[i] = clampd(ax[i] + g[i] - y[i] / sigma, cl[i], cu[i]);
+        for (int i = threadIdx.x; i < m; i += BT) apr[i] = clampd(ax[i] + g[i] - y[i] / sigma, cl[i], cu[i]);
         __syncthreads();
         C.project_affine(apr, C.cv(0));
     }
-    for (int i = threadIdx.x; i < m; i += kBT) {
+    for (int i = threadIdx.x; i < m; i += BT) {
         const double cur = y[i];
         const double v = ax[i] + g[i];
         const double pj = I.has_affine ? apr[i] : clampd(v - cur / sigma, cl[i], cu[i]);
@@ -453,16 +457,16 @@ template <class T> __device__ __noinline__ void pdhg_step(const Ctx<T> &C, State
 }
 
 // reference src/pdhg_core_op.cu:832-845
-template <class T> __device__ inline void halpern(const Ctx<T> &C, const State &S) {
+template <class T, int BT> __device__ inline void halpern(const Ctx<T, BT> &C, const State &S) {
     const double rho = C.P->rho;
     const double wgt = (double)(S.inner_count + 1) / (S.inner_count + 2);
     T *x = C.nv(vX), *y = C.mv(wY);
     const T *xr = C.nv(vXR), *x0 = C.nv(vX0), *yr = C.mv(wYR), *y0 = C.mv(wY0);
-    for (int i = threadIdx.x; i < C.I.n; i += kBT) {
+    for (int i = threadIdx.x; i < C.I.n; i += BT) {
         const double r = rho * xr[i] + (1.0 - rho) * x[i];
         x[i] = wgt * r + (1.0 - wgt) * x0[i];
     }
-    for (int i = threadIdx.x; i < C.I.m; i += kBT) {
+    for (int i = threadIdx.x; i < C.I.m; i += BT) {
         const double r = rho * yr[i] + (1.0 - rho) * y[i];
         y[i] = wgt * r + (1.0 - wgt) * y0[i];
     }
@@ -470,12 +474,12 @@ template <class T> __device__ inline void halpern(const Ctx<T> &C, const State &
 }
 
 // reference src/pdhg_core_op.cu:973-1035
-template <class T> __device__ __noinline__ void fixed_point_error(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ __noinline__ void fixed_point_error(const Ctx<T, BT> &C, State &S) {
     const int n = C.I.n, m = C.I.m;
     T *dx = C.nv(vDX), *dy = C.mv(wDY), *aty = C.nv(vATY);
     const T *xr = C.nv(vXR), *x = C.nv(vX), *yr = C.mv(wYR), *y = C.mv(wY);
-    for (int i = threadIdx.x; i < n; i += kBT) dx[i] = xr[i] - x[i];
-    for (int i = threadIdx.x; i < m; i += kBT) dy[i] = yr[i] - y[i];
+    for (int i = threadIdx.x; i < n; i += BT) dx[i] = xr[i] - x[i];
+    for (int i = threadIdx.x; i < m; i += BT) dy[i] = yr[i] - y[i];
     __syncthreads();
     C.At(dy, aty);
     const double dn = sqrt(C.red.dot(m, [&](int i) { return dy[i] * dy[i]; }));
@@ -488,7 +492,7 @@ template <class T> __device__ __noinline__ void fixed_point_error(const Ctx<T> &
 }
 
 // reference src/pdhg_core_op.cu:859-931
-template <class T> __device__ __noinline__ void restart(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ __noinline__ void restart(const Ctx<T, BT> &C, State &S) {
     const Params &P = *C.P;
     const int n = C.I.n, m = C.I.m;
     T *x0 = C.nv(vX0), *x = C.nv(vX), *y0 = C.mv(wY0), *y = C.mv(wY);
@@ -513,15 +517,15 @@ template <class T> __device__ __noinline__ void restart(const Ctx<T> &C, State &
         S.best_gap = gapv;
         S.best_pw = S.pw;
     }
-    for (int i = threadIdx.x; i < n; i += kBT) x0[i] = x[i] = xp[i];
-    for (int i = threadIdx.x; i < m; i += kBT) y0[i] = y[i] = yp[i];
+    for (int i = threadIdx.x; i < n; i += BT) x0[i] = x[i] = xp[i];
+    for (int i = threadIdx.x; i < m; i += BT) y0[i] = y[i] = yp[i];
     __syncthreads();
     S.inner_count = 0;
     S.last_trial_fpe = INFINITY;
 }
 
 // reference src/utils.cu:291-322
-template <class T> __device__ inline bool should_restart(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ inline bool should_restart(const Ctx<T, BT> &C, State &S) {
     const Params &P = *C.P;
     bool r = false;
     if (S.total == P.ef) r = true;
@@ -535,7 +539,7 @@ template <class T> __device__ inline bool should_restart(const Ctx<T> &C, State 
 }
 
 // KKT residuals (reference src/pdhg_core_op.cu:1037-1294).
-template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ __noinline__ void residual(const Ctx<T, BT> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     const bool linf = C.P->norm_inf != 0;
@@ -548,7 +552,7 @@ template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State 
     C.A(xp, ax);
     C.At(yp, aty);
     C.obj_product(xp, qx);
-    for (int i = threadIdx.x; i < m; i += kBT) {
+    for (int i = threadIdx.x; i < m; i += BT) {
         const double v = ax[i] + g[i];
         const double pj = clampd(v, cl[i], cu[i]);
         pres[i] = (v - pj) * cr[i];
@@ -556,7 +560,7 @@ template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State 
     }
     const bool lp = I.qkind == kQNone;
     double dres_acc = 0.0;
-    for (int j = threadIdx.x; j < n; j += kBT) {
+    for (int j = threadIdx.x; j < n; j += BT) {
         double dres;
         if (lp) {
             dres = (c[j] - aty[j] - ds[j]) * vr[j];
@@ -575,13 +579,13 @@ template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State 
     double amem_norm = 0.0, acomp_norm = 0.0;
     if (I.has_affine) {  // reference src/pdhg_core_op.cu:1125-1129,114-201
         double *acomp = C.cv(2);
-        for (int i = threadIdx.x; i < m; i += kBT) {
+        for (int i = threadIdx.x; i < m; i += BT) {
             pres[i] = clampd(ax[i] + g[i], cl[i], cu[i]);
             amem[i] = 0.0;
         }
         __syncthreads();
         C.project_affine(pres, C.cv(1));
-        for (int k = threadIdx.x; k < I.ncones; k += kBT) {
+        for (int k = threadIdx.x; k < I.ncones; k += BT) {
             const Cone cn = C.cones[k];
             const int L = (cn.type == CONE_STANDARD_SOC || cn.type == CONE_ROTATED_SOC) ? cn.vdim + 2 : 3;
             double dotv = 0.0;
@@ -593,7 +597,7 @@ template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State 
         }
         __syncthreads();
         C.project_affine(amem, C.cv(1));
-        for (int i = threadIdx.x; i < m; i += kBT) {
+        for (int i = threadIdx.x; i < m; i += BT) {
             pres[i] = (ax[i] + g[i] - pres[i]) * cr[i];
             amem[i] *= afd[i];
         }
@@ -626,18 +630,18 @@ template <class T> __device__ __noinline__ void residual(const Ctx<T> &C, State 
 }
 
 // Infeasibility rays (reference src/pdhg_core_op.cu:1296-1429).
-template <class T> __device__ __noinline__ void infeasibility(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ __noinline__ void infeasibility(const Ctx<T, BT> &C, State &S) {
     const Inst &I = C.I;
     const int n = I.n, m = I.m;
     T *dx = C.nv(vDX), *dy = C.mv(wDY), *ax = C.mv(wAX), *aty = C.nv(vATY), *qx = C.nv(vQX);
     const T *lb = C.nv(vLB), *ub = C.nv(vUB), *cl = C.mv(wCL), *cu = C.mv(wCU), *c = C.nv(vC),
                  *clf = C.mv(wCLF), *cuf = C.mv(wCUF), *g = C.mv(wG), *lbf = C.nv(vLBF), *ubf = C.nv(vUBF),
                  *cr = C.mv(wCR), *vr = C.nv(vVR);
-    for (int j = threadIdx.x; j < n; j += kBT) {
+    for (int j = threadIdx.x; j < n; j += BT) {
         if (isfinite(lb[j])) dx[j] = fmax(dx[j], 0.0);
         if (isfinite(ub[j])) dx[j] = fmin(dx[j], 0.0);
     }
-    for (int i = threadIdx.x; i < m; i += kBT) {
+    for (int i = threadIdx.x; i < m; i += BT) {
         if (!isfinite(cl[i])) dy[i] = fmin(dy[i], 0.0);
         if (!isfinite(cu[i])) dy[i] = fmax(dy[i], 0.0);
     }
@@ -645,7 +649,7 @@ template <class T> __device__ __noinline__ void infeasibility(const Ctx<T> &C, S
     const double prn = C.red.norm(n, true, [&](int j) { return dx[j]; });
     if (prn > 0.0) {
         const double sc = 1.0 / prn;
-        for (int j = threadIdx.x; j < n; j += kBT) dx[j] *= sc;
+        for (int j = threadIdx.x; j < n; j += BT) dx[j] *= sc;
         __syncthreads();
     }
     const double drn = C.red.norm(m, true, [&](int i) { return dy[i]; });
@@ -682,7 +686,7 @@ template <class T> __device__ __noinline__ void infeasibility(const Ctx<T> &C, S
 }
 
 // reference src/utils.cu:219-289 (the batch applies no wall-clock limit)
-template <class T> __device__ inline void check_termination(const Ctx<T> &C, State &S) {
+template <class T, int BT> __device__ inline void check_termination(const Ctx<T, BT> &C, State &S) {
     const Params &P = *C.P;
     if (S.rel_du < P.eps_feas && S.rel_pr < P.eps_feas && S.rel_gap < P.eps_opt) {
         S.reason = TERMINATION_REASON_OPTIMAL;
@@ -713,8 +717,8 @@ template <class T> __device__ inline void check_termination(const Ctx<T> &C, Sta
 // FP64 (dpool, wslabs), and a problem whose KKT error stops improving leaves
 // with kStalled; the host then widens the iterates into the FP64 slabs and
 // the FP64 kernel re-evaluates every problem before it terminates.
-template <class T>
-__global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const Inst *insts, const int *ipool,
+template <class T, int BT>
+__global__ void __launch_bounds__(BT) batch_solve_kernel(const Params *P, const Inst *insts, const int *ipool,
                                                           const T *vpool, const double *dpool, T *slabs,
                                                           double *wslabs, const Cone *cones, State *states, Out *out,
                                                           int budget) {
@@ -722,7 +726,7 @@ __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const
     State S = states[blockIdx.x];
     if (S.reason != TERMINATION_REASON_UNSPECIFIED) return;
     constexpr bool fp32 = !std::is_same<T, double>::value;
-    Ctx<T> C;
+    Ctx<T, BT> C;
     C.P = P;
     C.I = insts[blockIdx.x];
     C.arp = ipool + C.I.a_rp;
@@ -751,8 +755,12 @@ __global__ void __launch_bounds__(kBT) batch_solve_kernel(const Params *P, const
             if (!C.I.has_affine && S.is_major && tc < 3 * ef) infeasibility(C, S);
             check_termination(C, S);
             if (S.reason != TERMINATION_REASON_UNSPECIFIED) break;
-            if (fp32) {  // FP32 floor: hand the problem to the FP64 kernel
+            if (fp32) {  // near the tolerance or at the FP32 floor: hand the problem to the FP64 kernel
                 const double err = fmax(S.rel_pr, fmax(S.rel_du, S.rel_gap));
+                if (err < P->handover) {
+                    S.reason = kStalled;
+                    break;
+                }
                 if (err < 0.5 * S.best_err) {
                     S.best_err = err;
                     S.best_iter = tc;

@@ -124,3 +124,17 @@ def test_mixed_batch_limits_and_infeasibility():
     assert sols[0].status == "PRIMAL_INFEASIBLE"
     capped = pdhcg_amd.solve_batch([slow], precision=PRECISION_MIXED, eps=1e-12, iteration_limit=57)[0]
     assert capped.status == "ITERATION_LIMIT" and capped.iterations == 57
+
+
+@pytest.mark.parametrize("threads", ["256", "512"])
+def test_batch_thread_counts_match_oracle(threads, monkeypatch):
+    # both launch shapes the probe chooses between, pinned, to the same bar
+    monkeypatch.setenv("PDHCG_BATCH_THREADS", threads)
+    cases = [("lp", 60, 0), ("diag", 90, 1), ("lowrank_diag", 140, 0), ("lowrank_dense_d", 70, 0)]
+    problems = [workloads.random_problem(kind, seed=seed, n=n, m=max(10, n // 3)) for kind, n, seed in cases]
+    problems += workloads.portfolio_batch(2, seed=13, n_range=(50, 200), k_range=(10, 30))
+    for precision in (pdhcg_amd.PRECISION_FP64, PRECISION_MIXED):
+        sols = pdhcg_amd.solve_batch(problems, precision=precision, eps=EPS, iteration_limit=200000,
+                                     time_limit=60.0)
+        for i, (p, g) in enumerate(zip(problems, sols)):
+            _agree(p, g, oracle_solve(p, eps=EPS, iteration_limit=200000), affine=i >= len(cases))

@@ -82,3 +82,22 @@ def test_primal_only_start_solves_agree_with_cold(trained):
         assert abs(c.pobj - w.pobj) <= 5 * EPS * (1 + abs(c.pobj)), (c.pobj, w.pobj)
         k = kkt(p, w.x, w.y)
         assert k["rel_primal"] < 10 * EPS and k["rel_dual"] < 10 * EPS and k["rel_gap"] < 10 * EPS, k
+
+
+def test_warm_start_cuts_iterations_at_1e6_and_guard(trained):
+    # at the target tolerance: fewer iterations on average for the warm
+    # starts (single device solves of the same instances), and the guard
+    # keeps exactly the starts whose iteration-0 KKT error beats the cold one
+    fam, ws, Zte, pte = trained
+    sample = list(range(8))
+    cold = [pdhcg_amd.solve(pte[k], eps=EPS, iteration_limit=100000) for k in sample]
+    warm = [pdhcg_amd.solve(ws.start(pte[k], Zte[k]), eps=EPS, iteration_limit=100000) for k in sample]
+    assert all(c.status == w.status == "OPTIMAL" for c, w in zip(cold, warm))
+    assert np.mean([w.iterations for w in warm]) < np.mean([c.iterations for c in cold])
+    guarded, kept = ws.guarded([pte[k] for k in sample], Zte[sample], batch=False, eps=EPS)
+    e0 = lambda p: (lambda s: max(s.rel_primal, s.rel_dual, s.rel_gap))(
+        pdhcg_amd.solve(p, eps=EPS, iteration_limit=0))
+    for k, p, keep in zip(sample, guarded, kept):
+        w = ws.start(pte[k], Zte[k])
+        assert keep == (e0(w) < e0(pte[k]))
+        assert (p.primal_start is not None) == keep
