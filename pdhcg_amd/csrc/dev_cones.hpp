@@ -784,26 +784,27 @@ __global__ void __launch_bounds__(kBlock) power_violation_kernel(const X *xp, co
 // Warm start: each block keeps the eigenvector rows V' of its previous
 // projection (identity before the first) in its slice of the work buffer.
 // The sweeps start from B = V' A V'^T with V' as the initial rotation, so a
-// matrix close to the previous one is nearly diagonal already and needs two
-// or three sweeps instead of six to nine.  The result is the projection of A
-// whatever V' holds; V' only changes the rounding (the nearest PSD matrix is
-// unique).  A projection that does not converge resets V' to the identity.
+// matrix close to the previous one is nearly diagonal already and needs fewer
+// sweeps.  The result is the projection of A whatever V' holds; V' only
+// changes the rounding (the nearest PSD matrix is unique).  A projection that
+// does not converge resets V' to the identity.
 //
 // A sweep is kp - 1 rounds of the circle (round-robin tournament) ordering;
 // the kp / 2 pairs of a round are disjoint, so their rotations are applied at
 // once.  A round is two phases separated by barriers:
 //   1. each pair (p, q) gets its rotation (c, s) from its 2x2 diagonal block;
-//   2. every 2x2 block A[{p,q},{r,s}] (pair P, pair R) becomes J_P' B J_R,
-//      computed by the thread that owns the block in the orientation
-//      min(P, R), max(P, R), so the two mirror blocks hold bit-identical
-//      values and A stays exactly symmetric (a diagonal block's off-diagonal
-//      entry is set to 0); rows p, q of V' become J_P' V'[{p,q}, :].
-// A lives in LDS; V' too when both fit the workgroup's LDS (mode 0),
-// otherwise V' stays in the work buffer (mode 1); with neither fitting
-// (orders above about 140) both do (mode 2).  Sweeps run until the
-// off-diagonal mass is at most 1e-30 of the total.  A matrix not converged
-// after kPsdMaxSweeps sweeps (only non-finite input does that) is written as
-// NaN, as the reference writes NaN when its eigensolver fails (reference
+//   2. every 2x2 block A[{p,q},{r,s}] (pairs P <= R) becomes J_P' B J_R,
+//      computed by one thread; rows p, q of V' become J_P' V'[{p,q}, :].
+// The matrix is held as its packed lower triangle (entry (i, j), i >= j, at
+// i (i + 1) / 2 + j), so every entry belongs to exactly one block and is
+// read and written once per round; a diagonal block's off-diagonal entry is
+// set to 0.  Where the two live (LDS or the work buffer) is the block's mode:
+// 0 both in LDS, 1 A in LDS and V' in the work buffer, 2 V' in LDS and A in
+// the work buffer, 3 both in the work buffer.  The dense input and the
+// transform's temporary stay in the work buffer.  Sweeps run until the off-diagonal mass
+// is at most 1e-30 of the total.  A matrix not converged after kPsdMaxSweeps
+// sweeps (only non-finite input does that) is written as NaN, as the
+// reference writes NaN when its eigensolver fails (reference
 // src/kernels/pdhcg_psd_cone_kernels.cu:263-486).
 
 constexpr int kPsdMaxSweeps = 60;
@@ -815,6 +816,8 @@ constexpr int kPsdThreads = 512, kPsdThreadsLarge = 1024;
 template <int NT> struct PsdChunks {
     static constexpr int a = NT >= 1024 ? 2 : 4, v = NT >= 1024 ? 4 : 8;
 };
+__host__ __device__ inline bool psd_v_in_lds(int mode) { return mode == 0 || mode == 2; }
+__host__ __device__ inline bool psd_a_in_lds(int mode) { return mode == 0 || mode == 1; }
 
 // One PSD block of a launch.
 struct PsdBlock {
@@ -825,20 +828,21 @@ struct PsdBlock {
     long long work;  // offset (doubles) of the block's slice of the work buffer
 };
 
-// Workgroup LDS (dynamic) of a projection of order k in a mode: A and V'
-// (by mode), the round's rotations and pairs, and the clamped eigenvalues.
+// Workgroup LDS (dynamic) of a projection of order k in a mode: V' and the
+// packed A (by mode), the round's rotations and pairs, and the clamped
+// eigenvalues.
 __host__ __device__ inline size_t psd_lds_bytes(int k, int mode) {
-    const size_t kp = (size_t)(k + (k & 1)), np = kp / 2;
-    const size_t mats = mode == 0 ? 2 : (mode == 1 ? 1 : 0);
-    return (mats * kp * kp + 2 * np + kp) * sizeof(double) + 2 * np * sizeof(int);
+    const size_t kp = (size_t)(k + (k & 1)), np = kp / 2, kt = kp * (kp + 1) / 2;
+    const size_t mats = (psd_v_in_lds(mode) ? kp * kp : 0) + (psd_a_in_lds(mode) ? kt : 0);
+    return (mats + 2 * np + kp) * sizeof(double) + 2 * np * sizeof(int);
 }
 
 // Work doubles per block: the kept V' of the two projection sites (the
-// iterate and the dual residual), the transform's temporary and, in mode 2,
-// A itself.
+// iterate and the dual residual), the dense input, the transform's
+// temporary and the packed A when it is not in LDS.
 __host__ __device__ inline size_t psd_work_doubles(int k, int mode) {
-    const size_t kp = (size_t)(k + (k & 1));
-    return (mode == 2 ? 4 : 3) * kp * kp;
+    const size_t kp = (size_t)(k + (k & 1)), kt = kp * (kp + 1) / 2;
+    return 4 * kp * kp + (psd_a_in_lds(mode) ? 0 : kt);
 }
 
 // Row r and column c (r >= c) of svec slot `slot` of an order-k block
@@ -851,6 +855,19 @@ __device__ __forceinline__ void psd_slot_rc(int slot, int k, int &r, int &c) {
         ++c;
     }
     r = c + rem;
+}
+
+// Packed lower-triangle index of (i, j), either order.
+__device__ __forceinline__ int psd_tri(int i, int j) {
+    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+// Row of packed index e (the largest i with i (i + 1) / 2 <= e).
+__device__ __forceinline__ int psd_tri_row(int e) {
+    int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    if ((i + 1) * (i + 2) / 2 <= e) ++i;
+    if (i * (i + 1) / 2 > e) --i;
+    return i;
 }
 
 // Sets the kept V' of every block of a launch to the identity.
@@ -867,7 +884,8 @@ static __global__ void psd_identity_kernel(const PsdBlock *blocks, double *work)
 // 4 x 4 register tile of C = X Y over kp x kp row-major operands, rows
 // i0.., columns j0..; entries outside kp read as zero.  Lanes of a wave share
 // the tile row (X reads broadcast) and take consecutive column tiles (Y reads
-// contiguous).
+// contiguous); four steps of l are loaded before they are multiplied, so
+// the loads of a Y in the work buffer overlap.
 __device__ __forceinline__ void psd_tile_xy(const double *X, const double *Y, int kp, int i0, int j0,
                                             double acc[4][4]) {
     for (int u = 0; u < 4; ++u)
@@ -879,7 +897,27 @@ __device__ __forceinline__ void psd_tile_xy(const double *X, const double *Y, in
         yj[u] = j0 + u < kp;
         xr[u] = X + (size_t)(xi[u] ? i0 + u : i0) * kp;
     }
-    for (int l = 0; l < kp; ++l) {
+    constexpr int LU = 4;
+    int l = 0;
+    for (; l + LU <= kp; l += LU) {
+        double a[LU][4], b[LU][4];
+#pragma unroll
+        for (int d = 0; d < LU; ++d) {
+            const double *yl = Y + (size_t)(l + d) * kp + j0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[d][u] = xi[u] ? xr[u][l + d] : 0.0;
+                b[d][u] = yj[u] ? yl[u] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < LU; ++d)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) acc[u][w] = fma(a[d][u], b[d][w], acc[u][w]);
+    }
+    for (; l < kp; ++l) {
         const double *yl = Y + (size_t)l * kp + j0;
         double a[4], b[4];
         for (int u = 0; u < 4; ++u) {
@@ -908,20 +946,23 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
     }
     const int kp = k + (k & 1), np = kp / 2, m = kp - 1;
     const size_t kk = (size_t)kp * kp;
+    const int kt = kp * (kp + 1) / 2;
     double *wb = work + blk.work;
     double *Vkeep = wb + (size_t)site * kk;  // V' kept between calls at this site
-    double *S = wb + 2 * kk;                  // temporary of the warm transform
-    double *A = mode == 2 ? wb + 3 * kk : psd_lds;
-    double *Vt = mode == 0 ? psd_lds + kk : Vkeep;
-    double *cs = psd_lds + (mode == 0 ? 2 * kk : (mode == 1 ? kk : 0));
+    double *F = wb + 2 * kk;                  // dense A
+    double *S = wb + 3 * kk;                  // temporary of the warm transform
+    const bool vlds = psd_v_in_lds(mode), alds = psd_a_in_lds(mode);
+    double *Vt = vlds ? psd_lds : Vkeep;
+    double *A = alds ? psd_lds + (vlds ? kk : 0) : wb + 4 * kk;
+    double *cs = psd_lds + (vlds ? kk : 0) + (alds ? kt : 0);
     double *lam = cs + 2 * np;
     int *pq = reinterpret_cast<int *>(lam + kp);
     const int L = k * (k + 1) / 2;
 
-    // svec -> dense (off-diagonal entries / sqrt 2) with zero padding; V' in
+    // svec -> dense F (off-diagonal entries / sqrt 2) with zero padding; V' in
     for (size_t i = tid; i < kk; i += nt) {
-        A[i] = 0.0;
-        if (mode == 0) Vt[i] = Vkeep[i];
+        F[i] = 0.0;
+        if (vlds) Vt[i] = Vkeep[i];
     }
     __syncthreads();
     for (int slot = tid; slot < L; slot += nt) {
@@ -929,18 +970,18 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
         psd_slot_rc(slot, k, r, c);
         double val = (double)v[start + slot];
         if (r != c) val *= kInvSqrt2;
-        A[(size_t)r * kp + c] = val;
-        A[(size_t)c * kp + r] = val;
+        F[(size_t)r * kp + c] = val;
+        F[(size_t)c * kp + r] = val;
     }
     __syncthreads();
 
-    // warm transform B = V' A V'^T: S = (V' A)^T = A V'^T into the work
-    // buffer, then B = V' S on the lower tiles, mirrored (B is symmetric)
+    // warm transform B = V' F V'^T: S = (V' F)^T = F V'^T, then the lower
+    // triangle of B = V' S into the packed A
     const int nb = (kp + 3) / 4;
     for (int t = tid; t < nb * nb; t += nt) {
         const int ti = t / nb, tj = t - ti * nb;
         double acc[4][4];
-        psd_tile_xy(Vt, A, kp, 4 * ti, 4 * tj, acc);
+        psd_tile_xy(Vt, F, kp, 4 * ti, 4 * tj, acc);
         for (int u = 0; u < 4; ++u)
             for (int w = 0; w < 4; ++w)
                 if (4 * ti + u < kp && 4 * tj + w < kp) S[(size_t)(4 * tj + w) * kp + 4 * ti + u] = acc[u][w];
@@ -954,21 +995,24 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
         for (int u = 0; u < 4; ++u)
             for (int w = 0; w < 4; ++w) {
                 const int i = 4 * ti + u, j = 4 * tj + w;
-                if (i < kp && j <= i) {
-                    A[(size_t)i * kp + j] = acc[u][w];
-                    A[(size_t)j * kp + i] = acc[u][w];
-                }
+                if (i < kp && j <= i) A[i * (i + 1) / 2 + j] = acc[u][w];
             }
     }
     __syncthreads();
 
+    // phase-2 work split, fixed for the whole projection: threads per V'
+    // row pair (tpp) and per block row pair (tpr), and the strides that
+    // cover more pairs than threads
+    const int tpp = nt >= np ? nt / np : 1, pstride = nt / tpp;
+    const int nrp = (np + 1) / 2, tpr = nt >= nrp ? nt / nrp : 1, rstride = nt / tpr;
     bool converged = false;
     for (int sweep = 0; sweep <= kPsdMaxSweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
-        for (size_t i = tid; i < kk; i += nt) {
-            const double a = A[i];
-            tot += a * a;
-            if (i / kp != i % kp) off += a * a;
+        for (int e = tid; e < kt; e += nt) {
+            const double a = A[e];
+            const int i = psd_tri_row(e);
+            if (e - i * (i + 1) / 2 == i) tot += a * a;
+            else off += 2.0 * (a * a);
         }
         off = wave_sum(off);
         tot = wave_sum(tot);
@@ -983,6 +1027,7 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                 o += red[2 * w];
                 s += red[2 * w + 1];
             }
+            s += o;
             stop = (o <= 1e-30 * s || o == 0.0) ? 1 : 0;
         }
         __syncthreads();
@@ -1003,10 +1048,10 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                     b = (t - i + m) % m;
                 }
                 const int p = a < b ? a : b, q = a < b ? b : a;
-                const double apq = A[(size_t)p * kp + q];
+                const double apq = A[q * (q + 1) / 2 + p];
                 double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
-                    const double theta = (A[(size_t)q * kp + q] - A[(size_t)p * kp + p]) / (2.0 * apq);
+                    const double theta = (A[q * (q + 1) / 2 + q] - A[p * (p + 1) / 2 + p]) / (2.0 * apq);
                     const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                     c = 1.0 / sqrt(tt * tt + 1.0);
                     s = tt * c;
@@ -1017,86 +1062,112 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                 pq[2 * i + 1] = q;
             }
             __syncthreads();
-            // phase 2: every 2x2 block by its owner, then the rows of V'.
-            // Each thread loads kPsdChunk items before it stores any, so
-            // their LDS (or L2) round trips overlap instead of queueing
-            // behind the stores of the item before.
-            for (int w0 = tid; w0 < np * np; w0 += kPsdChunk * nt) {
-                int off[kPsdChunk][4];
-                double e[kPsdChunk][4], rot[kPsdChunk][4];
-                bool tr[kPsdChunk], diag[kPsdChunk], live[kPsdChunk];
+            // phase 2: V' row pairs and the 2x2 blocks.  The loads of this
+            // thread's first V' chunk are issued before the block updates, so
+            // their round trip (the work buffer when V' is not in LDS)
+            // overlaps them.
+            const int P0 = tid / tpp;
+            double c0 = 1.0, s0 = 0.0;
+            double *vp0 = Vt, *vq0 = Vt;
+            double va0[kPsdChunkV], vb0[kPsdChunkV];
+            if (P0 < np) {
+                c0 = cs[2 * P0];
+                s0 = cs[2 * P0 + 1];
+                vp0 = Vt + (size_t)pq[2 * P0] * kp;
+                vq0 = Vt + (size_t)pq[2 * P0 + 1] * kp;
+            }
 #pragma unroll
-                for (int j = 0; j < kPsdChunk; ++j) {
-                    const int w = w0 + j * nt;
-                    live[j] = w < np * np;
-                    const int P = live[j] ? w / np : 0, R = live[j] ? w - P * np : 0;
-                    const int p = pq[2 * P], q = pq[2 * P + 1], r = pq[2 * R], s2 = pq[2 * R + 1];
-                    off[j][0] = p * kp + r;
-                    off[j][1] = p * kp + s2;
-                    off[j][2] = q * kp + r;
-                    off[j][3] = q * kp + s2;
-                    // orientation (lo, hi): lo = min(P, R); b is the (lo, hi) block
-                    tr[j] = P > R;
-                    diag[j] = P == R;
-                    const int lo = tr[j] ? R : P, hi = tr[j] ? P : R;
-                    rot[j][0] = cs[2 * lo];
-                    rot[j][1] = cs[2 * lo + 1];
-                    rot[j][2] = cs[2 * hi];
-                    rot[j][3] = cs[2 * hi + 1];
-                    for (int u = 0; u < 4; ++u) e[j][u] = live[j] ? A[off[j][u]] : 0.0;
-                }
+            for (int j = 0; j < kPsdChunkV; ++j) {
+                const int i = tid % tpp + j * tpp;
+                const bool ld = P0 < np && s0 != 0.0 && i < kp;
+                va0[j] = ld ? vp0[i] : 0.0;
+                vb0[j] = ld ? vq0[i] : 0.0;
+            }
+            // The 2x2 blocks: a block row pair holds block rows
+            // rp and np - 1 - rp (np + 1 blocks; the middle row alone when np
+            // is odd); its tpr threads take every tpr-th block, loading
+            // kPsdChunk blocks before storing any so their round trips overlap.
+            for (int rp = tid / tpr; rp < nrp; rp += rstride) {
+                const int len = ((np & 1) && rp == np / 2) ? rp + 1 : np + 1;
+                for (int j0 = tid % tpr; j0 < len; j0 += kPsdChunk * tpr) {
+                    int off4[kPsdChunk][4], lo[kPsdChunk], hi[kPsdChunk];
+                    double e[kPsdChunk][4];
+                    bool live[kPsdChunk];
 #pragma unroll
-                for (int j = 0; j < kPsdChunk; ++j) {
-                    if (!live[j]) continue;
-                    const double c1 = rot[j][0], s1 = rot[j][1], c2 = rot[j][2], s2 = rot[j][3];
-                    const double b00 = e[j][0], b01 = tr[j] ? e[j][2] : e[j][1], b10 = tr[j] ? e[j][1] : e[j][2],
-                                 b11 = e[j][3];
-                    const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
-                    const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
-                    if (diag[j]) {
-                        A[off[j][0]] = c1 * m00 - s1 * m01;
-                        A[off[j][3]] = s1 * m10 + c1 * m11;
-                        A[off[j][1]] = 0.0;
-                        A[off[j][2]] = 0.0;
-                        continue;
+                    for (int j = 0; j < kPsdChunk; ++j) {
+                        const int jj = j0 + j * tpr;
+                        live[j] = jj < len;
+                        // block (hi, lo), hi >= lo: rows from pair lo, columns from pair hi
+                        hi[j] = !live[j] ? 0 : (jj <= rp ? rp : np - 1 - rp);
+                        lo[j] = !live[j] ? 0 : (jj <= rp ? jj : jj - rp - 1);
+                        const int p = pq[2 * lo[j]], q = pq[2 * lo[j] + 1], r = pq[2 * hi[j]], s2 = pq[2 * hi[j] + 1];
+                        off4[j][0] = psd_tri(p, r);
+                        off4[j][1] = psd_tri(p, s2);
+                        off4[j][2] = psd_tri(q, r);
+                        off4[j][3] = psd_tri(q, s2);
+                        for (int u = 0; u < 4; ++u) e[j][u] = live[j] ? A[off4[j][u]] : 0.0;
                     }
-                    const double n00 = c2 * m00 - s2 * m01, n01 = s2 * m00 + c2 * m01;
-                    const double n10 = c2 * m10 - s2 * m11, n11 = s2 * m10 + c2 * m11;
-                    A[off[j][0]] = n00;
-                    A[off[j][1]] = tr[j] ? n10 : n01;
-                    A[off[j][2]] = tr[j] ? n01 : n10;
-                    A[off[j][3]] = n11;
+#pragma unroll
+                    for (int j = 0; j < kPsdChunk; ++j) {
+                        if (!live[j]) continue;
+                        const double c1 = cs[2 * lo[j]], s1 = cs[2 * lo[j] + 1];
+                        const double c2 = cs[2 * hi[j]], s2 = cs[2 * hi[j] + 1];
+                        const double b00 = e[j][0], b01 = e[j][1], b10 = e[j][2], b11 = e[j][3];
+                        const double m00 = c1 * b00 - s1 * b10, m01 = c1 * b01 - s1 * b11;
+                        const double m10 = s1 * b00 + c1 * b10, m11 = s1 * b01 + c1 * b11;
+                        if (lo[j] == hi[j]) {
+                            // (p, q) = (r, s): entries (p,p), (q,p) (= off4 1 and 2), (q,q)
+                            A[off4[j][0]] = c1 * m00 - s1 * m01;
+                            A[off4[j][3]] = s1 * m10 + c1 * m11;
+                            A[off4[j][1]] = 0.0;
+                            continue;
+                        }
+                        A[off4[j][0]] = c2 * m00 - s2 * m01;
+                        A[off4[j][1]] = s2 * m00 + c2 * m01;
+                        A[off4[j][2]] = c2 * m10 - s2 * m11;
+                        A[off4[j][3]] = s2 * m10 + c2 * m11;
+                    }
                 }
             }
-            for (int w0 = tid; w0 < np * kp; w0 += kPsdChunkV * nt) {
-                int op[kPsdChunkV], oq[kPsdChunkV];
-                double va[kPsdChunkV], vb[kPsdChunkV], cc[kPsdChunkV], ss[kPsdChunkV];
+            if (P0 < np && s0 != 0.0) {
 #pragma unroll
                 for (int j = 0; j < kPsdChunkV; ++j) {
-                    const int w = w0 + j * nt;
-                    const bool live = w < np * kp;
-                    const int P = live ? w / kp : 0, i = live ? w - P * kp : 0;
-                    cc[j] = cs[2 * P];
-                    ss[j] = live ? cs[2 * P + 1] : 0.0;
-                    op[j] = pq[2 * P] * kp + i;
-                    oq[j] = pq[2 * P + 1] * kp + i;
-                    va[j] = ss[j] != 0.0 ? Vt[op[j]] : 0.0;
-                    vb[j] = ss[j] != 0.0 ? Vt[oq[j]] : 0.0;
+                    const int i = tid % tpp + j * tpp;
+                    if (i >= kp) continue;
+                    vp0[i] = c0 * va0[j] - s0 * vb0[j];
+                    vq0[i] = s0 * va0[j] + c0 * vb0[j];
                 }
+            }
+            // the rest of V': further column chunks of the first pair, then
+            // further pairs (tpp threads of a pair take every tpp-th column)
+            for (int P = P0; P < np; P += pstride) {
+                const double c = cs[2 * P], s = cs[2 * P + 1];
+                if (s == 0.0) continue;
+                double *vp = Vt + (size_t)pq[2 * P] * kp, *vq = Vt + (size_t)pq[2 * P + 1] * kp;
+                for (int i0 = tid % tpp + (P == P0 ? kPsdChunkV * tpp : 0); i0 < kp; i0 += kPsdChunkV * tpp) {
+                    double va[kPsdChunkV], vb[kPsdChunkV];
 #pragma unroll
-                for (int j = 0; j < kPsdChunkV; ++j) {
-                    if (ss[j] == 0.0) continue;
-                    Vt[op[j]] = cc[j] * va[j] - ss[j] * vb[j];
-                    Vt[oq[j]] = ss[j] * va[j] + cc[j] * vb[j];
+                    for (int j = 0; j < kPsdChunkV; ++j) {
+                        const int i = i0 + j * tpp;
+                        va[j] = i < kp ? vp[i] : 0.0;
+                        vb[j] = i < kp ? vq[i] : 0.0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kPsdChunkV; ++j) {
+                        const int i = i0 + j * tpp;
+                        if (i >= kp) continue;
+                        vp[i] = c * va[j] - s * vb[j];
+                        vq[i] = s * va[j] + c * vb[j];
+                    }
                 }
             }
             __syncthreads();
         }
     }
     // keep V' for the next call (identity after a failure)
-    if (mode == 0 || !converged)
+    if (vlds || !converged)
         for (size_t i = tid; i < kk; i += nt) Vkeep[i] = converged ? Vt[i] : (i / kp == i % kp ? 1.0 : 0.0);
-    for (int e = tid; e < kp; e += nt) lam[e] = fmax(A[(size_t)e * kp + e], 0.0);
+    for (int e = tid; e < kp; e += nt) lam[e] = fmax(A[e * (e + 1) / 2 + e], 0.0);
     __syncthreads();
     // svec of V' ^T max(diag(A), 0) V' on the lower 4 x 4 tiles of k x k
     const int nk = (k + 3) / 4;

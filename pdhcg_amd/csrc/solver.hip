@@ -290,8 +290,9 @@ class DevConeSet {
                          [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first > y.first; });
         if (!ps.empty()) {
             const size_t budget = psd_lds_budget();
-            // PDHCG_PSD_MODE=1|2 moves V' (1) or A and V' (2) to the work buffer
-            // on orders that would fit LDS (the tests run every mode)
+            // PDHCG_PSD_MODE=1|2|3 moves V' (1), A (2) or both (3) to the work
+            // buffer on orders whose layout would fit LDS (the tests run every
+            // mode)
             const char *fm = std::getenv("PDHCG_PSD_MODE");
             std::vector<PsdBlock> hb(ps.size());
             long long work = 0;
@@ -299,8 +300,12 @@ class DevConeSet {
                 PsdBlock &pb = hb[i];
                 pb.order = ps[i].first;
                 pb.start = ps[i].second;
-                pb.mode = psd_lds_bytes(pb.order, 0) <= budget ? 0 : (psd_lds_bytes(pb.order, 1) <= budget ? 1 : 2);
-                if (fm) pb.mode = std::max(pb.mode, std::min(2, std::atoi(fm)));
+                auto fits = [&](int md) { return psd_lds_bytes(pb.order, md) <= budget; };
+                pb.mode = fits(0) ? 0 : (fits(1) ? 1 : (fits(2) ? 2 : 3));
+                if (fm) {
+                    const int f = std::atoi(fm);
+                    if (f >= 1 && f <= 3) pb.mode = fits(f) ? f : 3;
+                }
                 pb.pad = 0;
                 pb.work = work;
                 work += (long long)psd_work_doubles(pb.order, pb.mode);

@@ -92,12 +92,12 @@ def test_projection_matches_oracle(t, uniform):
     assert np.array_equal(got[free], x[free])
 
 
-@pytest.mark.parametrize("mode", ["", "1", "2"])
+@pytest.mark.parametrize("mode", ["", "1", "2", "3"])
 def test_psd_large_orders_match_oracle(mode, monkeypatch):
     # orders 16 .. 128 (the conic-mix workload's range), odd and even, random,
     # clustered and rank-one spectra, several blocks of one order in one
-    # launch; mode "" lets the library keep A (and V' when it fits) in LDS,
-    # "1" / "2" force V' / both into the global work buffer
+    # launch; mode "" lets the library choose where V' and the packed A live,
+    # "1" / "2" / "3" force V' / A / both into the global work buffer
     if mode:
         monkeypatch.setenv("PDHCG_PSD_MODE", mode)
     rng = np.random.default_rng(77 + len(mode))
@@ -143,17 +143,17 @@ def _check_psd(got, blocks, specs, xs):
 
 def test_psd_mixed_orders_one_launch():
     # every order in one launch, largest first: orders 1..3, odd orders
-    # (padded), the LDS limits of both layouts (A and V' in LDS up to about
-    # 100, A alone up to about 140) and order 150, where both live in the
-    # global work buffer
+    # (padded), the LDS limits of the layouts (V' and the packed A in LDS up
+    # to order 114, one of them above) and order 150
     rng = np.random.default_rng(91)
-    blocks, specs, n = _psd_block_set(rng, [1, 2, 3, 16, 17, 64, 99, 100, 101, 128, 141, 142, 150, 7, 128])
+    blocks, specs, n = _psd_block_set(rng, [1, 2, 3, 16, 17, 64, 99, 113, 114, 115, 116, 128, 137, 138, 139, 140,
+                                            141, 150, 7, 128])
     x = np.concatenate([b for _, b in blocks])
     got = pdhcg_amd.project_cones(specs, x, np.ones(n))
     _check_psd(got, blocks, specs, x)
 
 
-@pytest.mark.parametrize("mode", ["", "1", "2"])
+@pytest.mark.parametrize("mode", ["", "1", "2", "3"])
 def test_psd_warm_sequence_matches_oracle(mode, monkeypatch):
     # one cone set projects a sequence: a point, two perturbations of it
     # (warm starts close to the answer), an unrelated point (a warm start far
