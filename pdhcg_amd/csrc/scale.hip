@@ -30,9 +30,18 @@ namespace pdhcg {
 namespace {
 
 // r-th row index of every stored entry of a CSR matrix
-__global__ void expand_rows_kernel(const int *rp, int rows, int *row_of) {
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x)
-        for (int e = rp[r]; e < rp[r + 1]; ++e) row_of[e] = r;
+// Row of every entry: a binary search of rp per entry (rp[lo] <= e <
+// rp[lo + 1]), so a row of any length costs no more than a short one.
+__global__ void expand_rows_kernel(const int *rp, int rows, int nnz, int *row_of) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += gridDim.x * blockDim.x) {
+        int lo = 0, hi = rows;  // rp[lo] <= e < rp[hi]
+        while (hi - lo > 1) {
+            const int mid = lo + (hi - lo) / 2;
+            if (rp[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        row_of[e] = lo;
+    }
 }
 
 __global__ void iota_kernel(int *v, int n) {
@@ -127,7 +136,8 @@ struct DevCsr {
         ci = dev_upload(h.ci);
         v = dev_upload(h.v);
         row_of = dev_alloc<int>((size_t)nnz);
-        if (rows > 0) hipLaunchKernelGGL(expand_rows_kernel, dim3(grid_for(rows)), dim3(256), 0, 0, rp, rows, row_of);
+        if (rows > 0 && nnz > 0)
+            hipLaunchKernelGGL(expand_rows_kernel, dim3(grid_for(nnz)), dim3(256), 0, 0, rp, rows, nnz, row_of);
         HIP_OK(hipGetLastError());
     }
     void release() {
@@ -261,9 +271,9 @@ class DevicePasses final : public MatrixPasses {
         void *scan_tmp = dev_alloc<char>(scan_bytes);
         HIP_OK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, counts, At_.rp, At_.rows + 1));
         (void)hipFree(scan_tmp);
-        if (At_.rows > 0)
-            hipLaunchKernelGGL(expand_rows_kernel, dim3(grid_for(At_.rows)), dim3(256), 0, 0, (const int *)At_.rp,
-                               At_.rows, At_.row_of);
+        if (At_.rows > 0 && nnz > 0)
+            hipLaunchKernelGGL(expand_rows_kernel, dim3(grid_for(nnz)), dim3(256), 0, 0, (const int *)At_.rp,
+                               At_.rows, nnz, At_.row_of);
         HIP_OK(hipGetLastError());
         HIP_OK(hipDeviceSynchronize());
         for (int *p : {keys_out, idx_in, idx_out, counts}) (void)hipFree(p);
