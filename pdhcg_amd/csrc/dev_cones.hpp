@@ -813,6 +813,9 @@ constexpr int kPsdMaxSweeps = 60;
 // any (phase 2); the larger workgroup hides more latency per round and fits
 // 128 VGPRs with the smaller chunks.
 constexpr int kPsdThreads = 512, kPsdThreadsLarge = 1024;
+// V' columns per wave when V' is held in registers during the sweeps (the
+// 1024-thread form up to order 128)
+constexpr int kPsdRegCols = 8, kPsdRegMinOrder = 96;
 template <int NT> struct PsdChunks {
     static constexpr int a = NT >= 1024 ? 2 : 4, v = NT >= 1024 ? 4 : 8;
 };
@@ -1005,6 +1008,27 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
     // cover more pairs than threads
     const int tpp = nt >= np ? nt / np : 1, pstride = nt / tpp;
     const int nrp = (np + 1) / 2, tpr = nt >= nrp ? nt / nrp : 1, rstride = nt / tpr;
+    // V' in registers (the 1024-thread form at orders 97 to 128): lane L of
+    // every wave holds rows lu, ld of V' (the pair at position L of the
+    // round robin) for its wave's kPsdRegCols columns.  After each round
+    // the rows move one position along the circle (the up rows one lane
+    // right, the down rows one lane left, position 0's up row fixed), the
+    // same pairing as the circle ordering, so V' needs no memory round trip
+    // in a round.  Elsewhere V' stays in Vt and pairs come from the formula
+    // (below order 97 the LDS update is the cheaper: the shifts cost the
+    // same whatever the order).
+    const bool vreg = NT >= kPsdThreadsLarge && np <= kWave && kp > kPsdRegMinOrder &&
+                      kp <= (NT / kWave) * kPsdRegCols;
+    const int lane = tid % kWave, wcol = (tid / kWave) * kPsdRegCols;
+    int lu = 2 * lane, ld = 2 * lane + 1;
+    double vu[kPsdRegCols], vd[kPsdRegCols];
+#pragma unroll
+    for (int j = 0; j < kPsdRegCols; ++j) {
+        const int c = wcol + j;
+        const bool ok = vreg && lane < np && c < kp;
+        vu[j] = ok ? Vt[(size_t)lu * kp + c] : 0.0;
+        vd[j] = ok ? Vt[(size_t)ld * kp + c] : 0.0;
+    }
     bool converged = false;
     for (int sweep = 0; sweep <= kPsdMaxSweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
@@ -1040,7 +1064,10 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
             // phase 1: the round's pairs and rotations
             for (int i = tid; i < np; i += nt) {
                 int a, b;
-                if (i == 0) {
+                if (vreg) {
+                    a = lu;  // i == lane: wave 0 holds position i
+                    b = ld;
+                } else if (i == 0) {
                     a = t;
                     b = m;
                 } else {
@@ -1062,27 +1089,8 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                 pq[2 * i + 1] = q;
             }
             __syncthreads();
-            // phase 2: V' row pairs and the 2x2 blocks.  The loads of this
-            // thread's first V' chunk are issued before the block updates, so
-            // their round trip (the work buffer when V' is not in LDS)
-            // overlaps them.
-            const int P0 = tid / tpp;
-            double c0 = 1.0, s0 = 0.0;
-            double *vp0 = Vt, *vq0 = Vt;
-            double va0[kPsdChunkV], vb0[kPsdChunkV];
-            if (P0 < np) {
-                c0 = cs[2 * P0];
-                s0 = cs[2 * P0 + 1];
-                vp0 = Vt + (size_t)pq[2 * P0] * kp;
-                vq0 = Vt + (size_t)pq[2 * P0 + 1] * kp;
-            }
-#pragma unroll
-            for (int j = 0; j < kPsdChunkV; ++j) {
-                const int i = tid % tpp + j * tpp;
-                const bool ld = P0 < np && s0 != 0.0 && i < kp;
-                va0[j] = ld ? vp0[i] : 0.0;
-                vb0[j] = ld ? vq0[i] : 0.0;
-            }
+            // phase 2: the 2x2 blocks, then the rows of V' (in Vt or in
+            // registers).
             // The 2x2 blocks: a block row pair holds block rows
             // rp and np - 1 - rp (np + 1 blocks; the middle row alone when np
             // is odd); its tpr threads take every tpr-th block, loading
@@ -1129,22 +1137,13 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                     }
                 }
             }
-            if (P0 < np && s0 != 0.0) {
-#pragma unroll
-                for (int j = 0; j < kPsdChunkV; ++j) {
-                    const int i = tid % tpp + j * tpp;
-                    if (i >= kp) continue;
-                    vp0[i] = c0 * va0[j] - s0 * vb0[j];
-                    vq0[i] = s0 * va0[j] + c0 * vb0[j];
-                }
-            }
-            // the rest of V': further column chunks of the first pair, then
-            // further pairs (tpp threads of a pair take every tpp-th column)
-            for (int P = P0; P < np; P += pstride) {
+            // rows p, q of V' in Vt: the tpp threads of a pair take every
+            // tpp-th column, kPsdChunkV columns loaded before any is stored
+            for (int P = vreg ? np : tid / tpp; P < np; P += pstride) {
                 const double c = cs[2 * P], s = cs[2 * P + 1];
                 if (s == 0.0) continue;
                 double *vp = Vt + (size_t)pq[2 * P] * kp, *vq = Vt + (size_t)pq[2 * P + 1] * kp;
-                for (int i0 = tid % tpp + (P == P0 ? kPsdChunkV * tpp : 0); i0 < kp; i0 += kPsdChunkV * tpp) {
+                for (int i0 = tid % tpp; i0 < kp; i0 += kPsdChunkV * tpp) {
                     double va[kPsdChunkV], vb[kPsdChunkV];
 #pragma unroll
                     for (int j = 0; j < kPsdChunkV; ++j) {
@@ -1161,8 +1160,52 @@ __global__ void __launch_bounds__(NT) psd_project_kernel(X *v, const PsdBlock *b
                     }
                 }
             }
+            if (vreg) {
+                // this round's rotation of the lane's two rows (p = min(lu, ld))
+                const double c = lane < np ? cs[2 * lane] : 1.0, s = lane < np ? cs[2 * lane + 1] : 0.0;
+                if (s != 0.0) {
+                    const bool up_is_p = lu < ld;
+#pragma unroll
+                    for (int j = 0; j < kPsdRegCols; ++j) {
+                        const double a = up_is_p ? vu[j] : vd[j], b = up_is_p ? vd[j] : vu[j];
+                        const double na = c * a - s * b, nb2 = s * a + c * b;
+                        vu[j] = up_is_p ? na : nb2;
+                        vd[j] = up_is_p ? nb2 : na;
+                    }
+                }
+                // next round's positions (a single pair stays in place);
+                // columns past kp are skipped by whole waves
+                const int from_prev = lane > 0 ? lane - 1 : 0, from_next = lane + 1 < kWave ? lane + 1 : lane;
+#pragma unroll
+                for (int j = 0; j < kPsdRegCols; ++j) {
+                    if (wcol + j >= kp) break;
+                    const double up_prev = __shfl(vu[j], from_prev, kWave);
+                    const double dn_next = __shfl(vd[j], from_next, kWave);
+                    const double dn0 = __shfl(vd[j], 0, kWave);
+                    const double nu = lane == 0 ? vu[j] : (lane == 1 ? dn0 : up_prev);
+                    vd[j] = lane == np - 1 ? (np > 1 ? vu[j] : vd[j]) : dn_next;
+                    vu[j] = nu;
+                }
+                const int lu_prev = __shfl(lu, from_prev, kWave), ld_next = __shfl(ld, from_next, kWave);
+                const int ld0 = __shfl(ld, 0, kWave);
+                const int nlu = lane == 0 ? lu : (lane == 1 ? ld0 : lu_prev);
+                ld = lane == np - 1 ? (np > 1 ? lu : ld) : ld_next;
+                lu = nlu;
+            }
             __syncthreads();
         }
+    }
+    if (vreg) {
+        // V' back to Vt, rows at the lanes' final positions
+#pragma unroll
+        for (int j = 0; j < kPsdRegCols; ++j) {
+            const int c = wcol + j;
+            if (lane < np && c < kp) {
+                Vt[(size_t)lu * kp + c] = vu[j];
+                Vt[(size_t)ld * kp + c] = vd[j];
+            }
+        }
+        __syncthreads();
     }
     // keep V' for the next call (identity after a failure)
     if (vlds || !converged)
