@@ -474,6 +474,23 @@ __device__ __forceinline__ void rx_rows(const V *T, int j0, int lane, const doub
     }
 }
 
+// Dense tiles of a CSR R whose rows have strictly increasing columns: entry
+// e (row found by a binary search of rp) goes to its tile slot, converted to
+// the storage type V; slots without an entry stay zero.
+template <class V>
+__global__ void r_tiles_kernel(const int *rp, const int *ci, const double *v, int k, long long nnz, V *Rd) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nnz;
+         e += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = k;  // rp[lo] <= e < rp[hi]
+        while (hi - lo > 1) {
+            const int mid = lo + (hi - lo) / 2;
+            if (rp[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        Rd[rx_tile_index(lo, ci[e], k)] = (V)v[e];
+    }
+}
+
 // Pass 1 of u = D R x: each wave owns one column tile (kRxWaveCols columns,
 // x for them held in registers), streams its rows eight at a time (32 loads
 // in flight per lane, one contiguous block per wave) and writes one partial

@@ -275,24 +275,37 @@ template <class V, class Acc> struct PanelSpmv {
     // rp/ci/v: host CSR of the operator (rows x cols).
     void build(int nrows, int ncols, const std::vector<int> &rp, const std::vector<int> &ci,
                const std::vector<double> &v) {
-        auto panel_of = [&](int col) { return (int)((long long)col * kPanels / std::max(ncols, 1)); };
+        // panel p holds columns [lo_p, lo_{p+1}) with lo_p the first column
+        // whose col * kPanels / ncols is p; one pass counts, one scatters
+        int lo[kPanels + 1];
+        for (int p = 0; p <= kPanels; ++p)
+            lo[p] = p == kPanels ? ncols : (int)(((long long)p * std::max(ncols, 1) + kPanels - 1) / kPanels);
+        auto panel_of = [&](int col) {
+            int p = 0;
+            while (p + 1 < kPanels && col >= lo[p + 1]) ++p;
+            return p;
+        };
         std::vector<std::vector<int>> prp(kPanels, std::vector<int>((size_t)nrows + 1, 0));
         for (int r = 0; r < nrows; ++r)
             for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e) prp[panel_of(ci[e])][(size_t)r + 1]++;
         for (int p = 0; p < kPanels; ++p)
             for (int r = 0; r < nrows; ++r) prp[p][(size_t)r + 1] += prp[p][r];
+        std::vector<std::vector<int>> pci(kPanels), fill(kPanels);
+        std::vector<std::vector<double>> pv(kPanels);
         for (int p = 0; p < kPanels; ++p) {
-            const size_t nnz = (size_t)prp[p][nrows];
-            std::vector<int> pci(nnz);
-            std::vector<double> pv(nnz);
-            std::vector<int> fill(prp[p].begin(), prp[p].end() - 1);
-            for (int r = 0; r < nrows; ++r)
-                for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e)
-                    if (panel_of(ci[e]) == p) {
-                        pci[fill[r]] = ci[e];
-                        pv[fill[r]++] = v[e];
-                    }
-            set.M[p] = make_binned<V>(nrows, ncols, prp[p], pci, pv);
+            pci[p].resize((size_t)prp[p][nrows]);
+            pv[p].resize((size_t)prp[p][nrows]);
+            fill[p].assign(prp[p].begin(), prp[p].end() - 1);
+        }
+        for (int r = 0; r < nrows; ++r)
+            for (int e = rp[r]; e < rp[(size_t)r + 1]; ++e) {
+                const int p = panel_of(ci[e]);
+                const int o = fill[p][r]++;
+                pci[p][o] = ci[e];
+                pv[p][o] = v[e];
+            }
+        for (int p = 0; p < kPanels; ++p) {
+            set.M[p] = make_binned<V>(nrows, ncols, prp[p], pci[p], pv[p]);
             max_blocks = std::max(max_blocks, set.M[p].total_blocks);
         }
         HIP_OK(hipMalloc(&set.partial, (size_t)kPanels * std::max(nrows, 1) * sizeof(Acc)));

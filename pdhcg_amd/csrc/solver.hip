@@ -82,7 +82,9 @@ template <class T> T *dalloc(size_t count) {
 
 template <class T, class S> T *dupload(const std::vector<S> &h, size_t count) {
     T *p = dalloc<T>(count);
-    if (count) {
+    if (count && std::is_same<T, S>::value) {
+        HIP_OK(hipMemcpy(p, h.data(), count * sizeof(T), hipMemcpyHostToDevice));
+    } else if (count) {
         std::vector<T> tmp(count);
         for (size_t i = 0; i < count; ++i) tmp[i] = (T)h[i];
         HIP_OK(hipMemcpy(p, tmp.data(), count * sizeof(T), hipMemcpyHostToDevice));
@@ -851,11 +853,34 @@ template <class V, class X> void Solver<V, X>::upload(const ScaledProblem &sp) {
         r_dense_ = dense_elems <= 4LL * sp.R.nnz() + 4096 && k_ <= kRxMaxRank;
         if (r_dense_) {
             rx_chunks_ = rx_chunks(n_);
-            std::vector<double> Rd((size_t)rx_chunks_ * kRxWaveCols * k_, 0.0);
-            for (int j = 0; j < k_; ++j)
-                for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e)
-                    Rd[rx_tile_index(j, sp.R.ci[e], k_)] += sp.R.v[e];
-            Rd_ = dupload<V>(Rd, Rd.size());
+            const size_t tiles = (size_t)rx_chunks_ * kRxWaveCols * k_;
+            // rows with strictly increasing columns (the usual case) are
+            // scattered into the tiles on the device; duplicate entries are
+            // summed in entry order on the host
+            bool unique = true;
+            for (int j = 0; j < k_ && unique; ++j)
+                for (int e = sp.R.rp[j] + 1; e < sp.R.rp[(size_t)j + 1]; ++e)
+                    if (sp.R.ci[e] <= sp.R.ci[e - 1]) {
+                        unique = false;
+                        break;
+                    }
+            const long long rnnz = sp.R.nnz();
+            if (unique && rnnz > 0) {
+                Rd_ = dzeros<V>(tiles);
+                int *drp = dupload<int>(sp.R.rp, sp.R.rp.size()), *dci = dupload<int>(sp.R.ci, (size_t)rnnz);
+                double *dv = dupload<double>(sp.R.v, (size_t)rnnz);
+                hipLaunchKernelGGL(r_tiles_kernel<V>, dim3(full_grid(rnnz)), dim3(kBlock), 0, nullptr,
+                                   (const int *)drp, (const int *)dci, (const double *)dv, k_, rnnz, Rd_);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipDeviceSynchronize());
+                for (void *q : {(void *)drp, (void *)dci, (void *)dv}) HIP_OK(hipFree(q));
+            } else {
+                std::vector<double> Rd(tiles, 0.0);
+                for (int j = 0; j < k_; ++j)
+                    for (int e = sp.R.rp[j]; e < sp.R.rp[(size_t)j + 1]; ++e)
+                        Rd[rx_tile_index(j, sp.R.ci[e], k_)] += sp.R.v[e];
+                Rd_ = dupload<V>(Rd, Rd.size());
+            }
             rx_blocks_ = std::max(1, (rx_chunks_ + kBlock / kWave - 1) / (kBlock / kWave));
             rx_partials_ = dzeros<double>((size_t)rx_chunks_ * k_);
             rx_sums_ = dzeros<double>((size_t)k_);
