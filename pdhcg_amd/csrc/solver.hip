@@ -332,9 +332,27 @@ class DevConeSet {
     // Projects v onto the scaled cones D K block by block (reference
     // src/cone_dispatch.cu:38-57).  residual_warm selects the warm starts of
     // the residual evaluation instead of those of the iteration.
+    // The PSD launch (at most one workgroup per PSD block, so most CUs are
+    // free) runs on a second stream beside the other cone lists: the solver
+    // enables this; the standalone cone entry points keep one stream.
+    void enable_overlap() {
+        if (side_ || !pset.count) return;
+        HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+    }
+
     template <class X>
     void project(X *v, const double *d, bool residual_warm, const int *gate, hipStream_t st) const {
         const int rw = residual_warm ? 1 : 0;
+        const bool overlap = side_ && pset.count;
+        if (overlap) {
+            // the PSD slots are disjoint from every other list's slots
+            HIP_OK(hipEventRecord(fork_, st));
+            HIP_OK(hipStreamWaitEvent(side_, fork_, 0));
+            psd_launch<X>(pset, v, 0, gate, side_);
+            HIP_OK(hipEventRecord(join_, side_));
+        }
         if (rsoc.count)
             hipLaunchKernelGGL((cone_project_kernel<X, CONE_ROTATED_SOC>), blocks(rsoc.count), dim3(kBlock), 0, st, v,
                                d, rsoc, rw, gate);
@@ -359,7 +377,8 @@ class DevConeSet {
         if (lsec.count)
             hipLaunchKernelGGL(section_long_project_kernel<X>, dim3(lsec.count), dim3(kBlock), 0, st, v, d,
                                (const X *)nullptr, 0.0, fixed, lsec, lsec_scratch, lsec_off, gate);
-        psd_launch<X>(pset, v, 0, gate, st);
+        if (overlap) HIP_OK(hipStreamWaitEvent(st, join_, 0));
+        else psd_launch<X>(pset, v, 0, gate, st);
     }
 
     // Cone step of the diagonal-Q prox: projection onto diag(d) K in the
@@ -432,6 +451,14 @@ class DevConeSet {
     }
 
     void release() {
+        if (side_) {
+            (void)hipStreamSynchronize(side_);
+            (void)hipStreamDestroy(side_);
+            (void)hipEventDestroy(fork_);
+            (void)hipEventDestroy(join_);
+            side_ = nullptr;
+            fork_ = join_ = nullptr;
+        }
         if (arena_) {  // arena memory belongs to the caller
             *this = DevConeSet::Cleared();
             return;
@@ -466,6 +493,8 @@ class DevConeSet {
     static dim3 blocks(int count) { return dim3((unsigned)((count + kBlock - 1) / kBlock)); }
 
     DevArena *arena_ = nullptr;
+    hipStream_t side_ = nullptr;  // the PSD stream of enable_overlap()
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
     struct Cleared {};
     DevConeSet &operator=(Cleared) {
         psd = power = false;
@@ -1028,6 +1057,7 @@ template <class V, class X> void Solver<V, X>::setup_cones(const ScaledProblem &
     const cone_blocks_t *b = sp.cones;
     if (!b || b->num_cones <= 0) return;
     cones_.build(b, n_);
+    cones_.enable_overlap();
     // fixed slots keep their start value (reference src/solver_state.cu:650-651,673)
     auto free_slot = [&](int i) { return !(b->is_fixed && b->is_fixed[i]); };
     for (int k = 0; k < b->num_cones; ++k) {
