@@ -170,6 +170,24 @@ class Desc:
         return C.byref(self.desc)
 
 
+def desc_diag(dvec) -> Desc:
+    """CSR descriptor of diag(dvec) built directly with numpy, with the
+    entries scipy's diags(dvec).tocsr() keeps (the nonzero ones)."""
+    v = np.asarray(dvec, dtype=np.float64)
+    keepm = v != 0.0
+    rp = np.zeros(v.size + 1, dtype=np.int32)
+    np.cumsum(keepm, out=rp[1:])
+    ci = np.ascontiguousarray(np.nonzero(keepm)[0], dtype=np.int32)
+    vals = np.ascontiguousarray(v[keepm])
+    d = matrix_desc_t()
+    d.m = d.n = int(v.size)
+    d.fmt = MATRIX_CSR
+    d.data.csr.nnz = int(vals.size)
+    d.data.csr.row_ptr, d.data.csr.col_ind, d.data.csr.vals = _ip(rp), _ip(ci), _dp(vals)
+    d.zero_tolerance = 0.0
+    return Desc(d, [rp, ci, vals])
+
+
 def desc_from(mat, fmt: str = "csr", zero_tolerance: float = 0.0) -> Optional[Desc]:
     """Descriptor from a numpy array (dense) or a scipy sparse matrix.
 

@@ -87,9 +87,14 @@ def build(lib, prefix: str, p: QPData) -> Optional[Handle]:
         vecs[name] = v
         keep.append(v)
     D = p.D
+    d_desc = None
     if D is not None and np.ndim(D) == 1:
-        import scipy.sparse as sp
-        D = sp.diags(np.asarray(D, dtype=np.float64)).tocsr()
+        if p.fmt.get("D", "csr") == "csr":
+            d_desc = capi.desc_diag(D)  # diag(D) without scipy's diags (many small problems)
+            keep.append(d_desc)
+        else:
+            import scipy.sparse as sp
+            D = sp.diags(np.asarray(D, dtype=np.float64)).tocsr()
     const = C.c_double(p.objective_constant)
     cones = capi.cone_specs(p.cones) if p.cones else None
     acones = capi.cone_specs(p.affine_cones) if p.affine_cones else None
@@ -97,7 +102,7 @@ def build(lib, prefix: str, p: QPData) -> Optional[Handle]:
     f = p.fmt
     ptr = getattr(lib, prefix + "create_qp_problem")(
         capi.as_ptr(c), _desc(p.Q, f.get("Q", "csr"), keep), _desc(p.R, f.get("R", "csr"), keep),
-        _desc(D, f.get("D", "csr"), keep), _desc(p.A, f.get("A", "csr"), keep),
+        d_desc.ptr if d_desc is not None else _desc(D, f.get("D", "csr"), keep), _desc(p.A, f.get("A", "csr"), keep),
         capi.as_ptr(vecs["con_lb"]), capi.as_ptr(vecs["con_ub"]),
         capi.as_ptr(vecs["var_lb"]), capi.as_ptr(vecs["var_ub"]), C.byref(const),
         len(p.cones), cones, _desc(p.F, f.get("F", "csr"), keep), capi.as_ptr(vecs["g"]),
