@@ -491,8 +491,9 @@ def main():
     ap.add_argument("--warm-train", type=int, default=4096, metavar="N",
                     help="training instances of the learned warm start (0: skip it)")
     ap.add_argument("--warm-test", type=int, default=256, metavar="N")
-    ap.add_argument("--conic-seconds", type=float, default=40.0,
-                    help="time limit of each conic-mix solve (0: skip the conic mix)")
+    ap.add_argument("--conic-seconds", type=float, default=25.0,
+                    help="time limit of each conic-mix solve, checked every 10 outer iterations "
+                         "(0: skip the conic mix)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched portfolio section (prints {\"batched\": ...})")
     ap.add_argument("--conic-only", action="store_true",
