@@ -565,14 +565,38 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                                    d_ipool, (const double *)d_vpool, (const double *)d_vpool, d_slabs, d_wslabs,
                                    d_cones, d_states, d_out, budget);
         };
-        // Threads per problem (the autotuned launch shape): both sizes run
-        // the same short probe from the same saved state (loop states, slabs
-        // and FP64 slabs restored after each), and the faster one runs the
-        // batch.  PDHCG_BATCH_THREADS=256|512 skips the probe.
-        int bt = kBT;
-        if (const char *e = std::getenv("PDHCG_BATCH_THREADS")) {
-            bt = std::atoi(e) == kBTSmall ? kBTSmall : kBT;
-        } else if (window > 1) {
+        // Launch-shape autotuning.  Each candidate runs the same short probe
+        // from the same saved state (loop states, slabs and FP64 slabs are
+        // copied device-to-device and restored after each), and the batch
+        // runs with the fastest, kept only when 5% faster than the default:
+        //   threads per problem, 512 or 256;
+        //   then, when some problem's A or A' has rows of 4 to 63 entries on
+        //   average (either form could win), the row length from which a
+        //   product takes a wavefront per row instead of a thread per row:
+        //   16 (default), 4 or 64.
+        // PDHCG_BATCH_THREADS=256|512 and PDHCG_BATCH_ROWS=4|16|64 pin them.
+        int bt = kBT, row_len = 16;
+        const char *et = std::getenv("PDHCG_BATCH_THREADS");
+        const char *er = std::getenv("PDHCG_BATCH_ROWS");
+        if (et) bt = std::atoi(et) == kBTSmall ? kBTSmall : kBT;
+        if (er && (std::atoi(er) == 4 || std::atoi(er) == 64)) row_len = std::atoi(er);
+        auto set_rows = [&](int len) {  // the per-problem row forms for a threshold, on the device
+            for (Inst &I : insts) {
+                const long long nnz = (long long)I.at_rp - I.a_ci;
+                I.a_long = nnz >= (long long)len * std::max(I.m, 1);
+                I.at_long = nnz >= (long long)len * std::max(I.n, 1);
+            }
+            HIP_OK(hipMemcpyAsync(d_insts, insts.data(), (size_t)B * sizeof(Inst), hipMemcpyHostToDevice, st));
+        };
+        bool ambiguous = false;
+        for (const Inst &I : insts) {
+            const long long nnz = (long long)I.at_rp - I.a_ci;
+            for (int rows : {I.m, I.n})
+                if (rows > 0 && nnz >= 4LL * rows && nnz < 64LL * rows) ambiguous = true;
+        }
+        if (row_len != 16) set_rows(row_len);
+        const bool tune_bt = !et && window > 1, tune_rows = !er && ambiguous && window > 1;
+        if (tune_bt || tune_rows) {
             const int probe = std::min(window, 25);
             const size_t sbytes = (size_t)soff[B] * (mixed ? sizeof(float) : sizeof(double));
             void *slab_src = mixed ? (void *)d_slabs32 : (void *)d_slabs;
@@ -594,26 +618,47 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             hipEvent_t e0, e1;
             HIP_OK(hipEventCreate(&e0));
             HIP_OK(hipEventCreate(&e1));
-            float ms[2] = {0.0f, 0.0f};
-            const int sizes[2] = {kBT, kBTSmall};
-            copy_all(true);
-            for (int c = 0; c < 2; ++c) {
+            auto timed = [&](int threads) {  // one probe from the saved state, restored afterwards
+                float ms = 0.0f;
                 HIP_OK(hipEventRecord(e0, st));
-                if (mixed) launch32(sizes[c], probe);
-                else launch64(sizes[c], probe);
+                if (mixed) launch32(threads, probe);
+                else launch64(threads, probe);
                 HIP_OK(hipEventRecord(e1, st));
                 HIP_OK(hipEventSynchronize(e1));
                 HIP_OK(hipGetLastError());
-                HIP_OK(hipEventElapsedTime(&ms[c], e0, e1));
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
                 copy_all(false);
+                return ms;
+            };
+            copy_all(true);
+            if (tune_bt) {
+                const float m512 = timed(kBT), m256 = timed(kBTSmall);
+                bt = m256 < 0.95f * m512 ? kBTSmall : kBT;
+                if (P.verbose >= 1)
+                    std::fprintf(stderr,
+                                 "[batch] threads per problem: %d (probe of %d passes: 512 -> %.1f ms, 256 -> %.1f ms)\n",
+                                 bt, probe, m512, m256);
+            }
+            if (tune_rows) {
+                const int lens[3] = {16, 4, 64};
+                float ms[3];
+                for (int c = 0; c < 3; ++c) {
+                    set_rows(lens[c]);
+                    ms[c] = timed(bt);
+                }
+                int best = 0;
+                for (int c = 1; c < 3; ++c)
+                    if (ms[c] < 0.95f * ms[0] && ms[c] < ms[best]) best = c;
+                row_len = lens[best];
+                set_rows(row_len);
+                if (P.verbose >= 1)
+                    std::fprintf(stderr,
+                                 "[batch] wavefront rows from %d entries (probe: 16 -> %.1f ms, 4 -> %.1f ms, 64 -> %.1f ms)\n",
+                                 row_len, ms[0], ms[1], ms[2]);
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             HIP_OK(hipStreamSynchronize(st));
-            bt = ms[1] < 0.95f * ms[0] ? kBTSmall : kBT;
-            if (P.verbose >= 1)
-                std::fprintf(stderr, "[batch] threads per problem: %d (probe of %d passes: 512 -> %.1f ms, 256 -> %.1f ms)\n",
-                             bt, probe, ms[0], ms[1]);
         }
         if (mixed) {
             run_phase([&] { launch32(bt, window); }, "FP32", INT_MAX);

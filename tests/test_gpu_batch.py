@@ -138,3 +138,19 @@ def test_batch_thread_counts_match_oracle(threads, monkeypatch):
                                      time_limit=60.0)
         for i, (p, g) in enumerate(zip(problems, sols)):
             _agree(p, g, oracle_solve(p, eps=EPS, iteration_limit=200000), affine=i >= len(cases))
+
+
+@pytest.mark.parametrize("rows", ["4", "16", "64"])
+def test_batch_row_forms_match_oracle(rows, monkeypatch):
+    # the three row-length thresholds the launch probe chooses between
+    # (a wavefront or a thread per row of A and A'), pinned, to the same bar;
+    # rows of 8 to 32 entries on average make each threshold split differently
+    monkeypatch.setenv("PDHCG_BATCH_ROWS", rows)
+    cases = [("lp", 120, 3), ("diag", 150, 4), ("lowrank_diag", 160, 5)]
+    problems = [workloads.random_problem(kind, seed=seed, n=n, m=max(10, n // 3)) for kind, n, seed in cases]
+    problems += workloads.portfolio_batch(2, seed=17, n_range=(60, 150), k_range=(10, 40))
+    for precision in (pdhcg_amd.PRECISION_FP64, PRECISION_MIXED):
+        sols = pdhcg_amd.solve_batch(problems, precision=precision, eps=EPS, iteration_limit=200000,
+                                     time_limit=60.0)
+        for i, (p, g) in enumerate(zip(problems, sols)):
+            _agree(p, g, oracle_solve(p, eps=EPS, iteration_limit=200000), affine=i >= len(cases))
