@@ -14,9 +14,12 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pdhcg.h"
@@ -324,6 +327,33 @@ template <class F> void parallel_for(int count, F f) {
     for (std::thread &t : pool) t.join();
 }
 
+// Launch shapes chosen by the probe, per batch shape, for the process.
+struct ShapeKey {
+    int mixed, count;
+    long long n, m, nnz;
+    bool operator<(const ShapeKey &o) const {
+        return std::tie(mixed, count, n, m, nnz) < std::tie(o.mixed, o.count, o.n, o.m, o.nnz);
+    }
+};
+struct ShapeChoice {
+    int threads, row_len;
+};
+std::mutex g_shape_mu;
+std::map<ShapeKey, ShapeChoice> g_shapes;
+
+bool lookup_shape(const ShapeKey &k, ShapeChoice &out) {
+    std::lock_guard<std::mutex> lock(g_shape_mu);
+    auto it = g_shapes.find(k);
+    if (it == g_shapes.end()) return false;
+    out = it->second;
+    return true;
+}
+
+void store_shape(const ShapeKey &k, const ShapeChoice &c) {
+    std::lock_guard<std::mutex> lock(g_shape_mu);
+    g_shapes[k] = c;
+}
+
 }  // namespace
 }  // namespace batch
 }  // namespace pdhcg
@@ -575,6 +605,9 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         //   product takes a wavefront per row instead of a thread per row:
         //   16 (default), 4 or 64.
         // PDHCG_BATCH_THREADS=256|512 and PDHCG_BATCH_ROWS=4|16|64 pin them.
+        // The choice is kept per process for the batch's shape (precision,
+        // problem count, total n, m and nnz(A)), so a family solved again
+        // (cold, warm, at another tolerance) is not probed again.
         int bt = kBT, row_len = 16;
         const char *et = std::getenv("PDHCG_BATCH_THREADS");
         const char *er = std::getenv("PDHCG_BATCH_ROWS");
@@ -594,8 +627,19 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             for (int rows : {I.m, I.n})
                 if (rows > 0 && nnz >= 4LL * rows && nnz < 64LL * rows) ambiguous = true;
         }
+        long long tot_n = 0, tot_m = 0, tot_nnz = 0;
+        for (const Inst &I : insts) {
+            tot_n += I.n;
+            tot_m += I.m;
+            tot_nnz += (long long)I.at_rp - I.a_ci;
+        }
+        const ShapeKey key{mixed ? 1 : 0, B, tot_n, tot_m, tot_nnz};
+        ShapeChoice known{};
+        const bool have = lookup_shape(key, known);
+        if (have && !et) bt = known.threads;
+        if (have && !er) row_len = known.row_len;
         if (row_len != 16) set_rows(row_len);
-        const bool tune_bt = !et && window > 1, tune_rows = !er && ambiguous && window > 1;
+        const bool tune_bt = !et && !have && window > 1, tune_rows = !er && !have && ambiguous && window > 1;
         if (tune_bt || tune_rows) {
             const int probe = std::min(window, 25);
             const size_t sbytes = (size_t)soff[B] * (mixed ? sizeof(float) : sizeof(double));
@@ -659,6 +703,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             HIP_OK(hipStreamSynchronize(st));
+            if (!et && !er) store_shape(key, ShapeChoice{bt, row_len});  // only fully probed choices
         }
         if (mixed) {
             run_phase([&] { launch32(bt, window); }, "FP32", INT_MAX);
