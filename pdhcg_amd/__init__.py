@@ -93,7 +93,14 @@ def solve_batch(problems, precision: int = PRECISION_FP64, **params) -> list:
     evaluates and terminates every problem in FP64."""
     L = lib()
     L.pdhcg_set_precision(precision)
-    handles = [build(L, "", p) for p in problems]
+    if len(problems) >= 64:
+        # the library's create_qp_problem copies run outside the GIL, so a
+        # pool of threads overlaps them across problems
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            handles = list(ex.map(lambda p: build(L, "", p), problems))
+    else:
+        handles = [build(L, "", p) for p in problems]
     try:
         count = len(problems)
         ptrs = (C.POINTER(capi.qp_problem_t) * max(count, 1))()
