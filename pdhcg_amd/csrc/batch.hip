@@ -641,7 +641,7 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
         if (row_len != 16) set_rows(row_len);
         const bool tune_bt = !et && !have && window > 1, tune_rows = !er && !have && ambiguous && window > 1;
         if (tune_bt || tune_rows) {
-            const int probe = std::min(window, 25);
+            const int probe = std::min(window, 12);
             const size_t sbytes = (size_t)soff[B] * (mixed ? sizeof(float) : sizeof(double));
             void *slab_src = mixed ? (void *)d_slabs32 : (void *)d_slabs;
             void *s_copy = nullptr, *w_copy = nullptr, *st_copy = nullptr;
