@@ -659,9 +659,16 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                 cp(d_wslabs, w_copy, (size_t)woff[B] * sizeof(double));
                 cp(d_states, st_copy, (size_t)B * sizeof(State));
             };
-            hipEvent_t e0, e1;
-            HIP_OK(hipEventCreate(&e0));
-            HIP_OK(hipEventCreate(&e1));
+            struct EventPair {  // destroyed on every exit, exceptions included
+                hipEvent_t a = nullptr, b = nullptr;
+                ~EventPair() {
+                    if (a) (void)hipEventDestroy(a);
+                    if (b) (void)hipEventDestroy(b);
+                }
+            } ev;
+            HIP_OK(hipEventCreate(&ev.a));
+            HIP_OK(hipEventCreate(&ev.b));
+            hipEvent_t e0 = ev.a, e1 = ev.b;
             auto timed = [&](int threads) {  // one probe from the saved state, restored afterwards
                 float ms = 0.0f;
                 HIP_OK(hipEventRecord(e0, st));
@@ -700,8 +707,6 @@ extern "C" int pdhcg_solve_batch(const qp_problem_t *const *problems, int count,
                                  "[batch] wavefront rows from %d entries (probe: 16 -> %.1f ms, 4 -> %.1f ms, 64 -> %.1f ms)\n",
                                  row_len, ms[0], ms[1], ms[2]);
             }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
             HIP_OK(hipStreamSynchronize(st));
             if (!et && !er) store_shape(key, ShapeChoice{bt, row_len});  // only fully probed choices
         }
