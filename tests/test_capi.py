@@ -262,3 +262,16 @@ def test_unsupported_entry_points_return_null():
     L.solve_qp_problem_distributed.argtypes = [C.c_void_p, C.c_void_p]
     assert not L.solve_qp_problem_distributed(None, None)
     assert not L.solve_qp_problem(None, None)
+
+
+def test_desc_diag_matches_scipy_diags():
+    # the numpy-built descriptor of a diagonal D holds the entries scipy's
+    # diags(D).tocsr() keeps (nonzero values, NaN kept), in the same order
+    import scipy.sparse as sp
+    from pdhcg_amd import capi
+    rng = np.random.default_rng(5)
+    for dv in (np.array([1.0, 0.0, 2.0, np.nan, -3.0]), np.array([3.0]), np.zeros(4), rng.uniform(-1, 1, 57)):
+        rp, ci, v = capi.desc_diag(dv).keep
+        m = sp.diags(dv).tocsr()
+        assert np.array_equal(rp, m.indptr) and np.array_equal(ci, m.indices)
+        assert np.array_equal(v, m.data, equal_nan=True)
